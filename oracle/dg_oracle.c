@@ -518,11 +518,16 @@ static void flux_hllc(double gam, const double* UL, const double* UR,
 
 /* Roe flux: absent from the reference (only a commented call site,
  * src/physics/euler/euler.cpp:846-857).  New: Roe-Pike wave
- * decomposition (Toro, Riemann Solvers, 3rd ed., sec. 11.2-11.3) with
- * Harten's entropy fix on the two acoustic waves, delta = 0.1*(|u~n|+c~).
+ * decomposition (Toro, Riemann Solvers, 3rd ed., sec. 11.2-11.3) with a
+ * Harten-Hyman-type entropy fix on the two acoustic waves:
+ *   delta = max(0, lam~ - lam(UL), lam(UR) - lam~),
+ *   |lam~| -> (lam~^2 + delta^2) / (2 delta)  when |lam~| < delta.
+ * delta > 0 only across an expansive (transonic rarefaction) wave, so a
+ * single shock -- including a stationary one -- stays exact.
  * Same convention as the reference fluxes: N is area weighted, the result
  * is |N| * Fhat(n_hat). */
-static double harten(double lam, double delta) {
+static double entropy_fix(double lam, double lamL, double lamR) {
+    double delta = fmax(0., fmax(lam - lamL, lamR - lam));
     double a = fabs(lam);
     return a < delta ? 0.5 * (lam * lam + delta * delta) / delta : a;
 }
@@ -553,10 +558,10 @@ static void flux_roe(double gam, const double* UL, const double* UR,
     const double a1 = (dp - rt * ct * dun) / (2. * c2);
     const double a5 = (dp + rt * ct * dun) / (2. * c2);
     const double a2 = drho - dp / c2;
-    const double delta = 0.1 * (fabs(unt) + ct);
-    const double l1 = harten(unt - ct, delta);
+    const double cL = sqrt(gam * pL / rL), cR = sqrt(gam * pR / rR);
+    const double l1 = entropy_fix(unt - ct, unL - cL, unR - cR);
     const double l2 = fabs(unt);
-    const double l5 = harten(unt + ct, delta);
+    const double l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
     double D[NS];
     /* acoustic waves */
     D[0] = l1 * a1 + l5 * a5;

@@ -1,0 +1,138 @@
+"""Pins of the oracle's mesh conventions, mass matrices, residual
+invariants and the reference's end-to-end 3D isentropic-vortex regression."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def _solver(**kw):
+    prm = O.make_params(**kw)
+    return O.Solver(prm)
+
+
+def test_face_table_and_orientations():
+    s = _solver(order=2, nelem=(3, 2, 4))
+    t = s.face_table()
+    assert t.shape == (3 * 24, 8)
+    # [rankL, elemL, faceL, orientL, rankR, elemR, faceR, orientR]
+    assert np.all(t[:, 0] == 0) and np.all(t[:, 4] == 0) and np.all(t[:, 3] == 0)
+    d = np.arange(len(t)) % 3
+    assert np.all(t[d == 0, 2] == 2) and np.all(t[d == 0, 6] == 4)   # x+ | x-
+    assert np.all(t[d == 1, 2] == 3) and np.all(t[d == 1, 6] == 1)   # y+ | y-
+    assert np.all(t[d == 2, 2] == 5) and np.all(t[d == 2, 6] == 0)   # z+ | z-
+    # orientation codes derived by matching points (shape.cpp:326-428)
+    assert np.all(t[d == 0, 7] == 5) and np.all(t[d == 1, 7] == 5) and np.all(t[d == 2, 7] == 4)
+    # every element appears exactly 3 times as L and 3 times as R
+    assert np.all(np.bincount(t[:, 1], minlength=24) == 3)
+    assert np.all(np.bincount(t[:, 5], minlength=24) == 3)
+
+
+def test_orientation_permutations():
+    # orientation 5 reverses each row, 4 transposes (shape.cpp:372-396)
+    m = 5
+    o5 = O.face_pts_order(5, m * m).reshape(m, m)
+    o4 = O.face_pts_order(4, m * m).reshape(m, m)
+    idx = np.arange(m * m).reshape(m, m)
+    assert np.array_equal(o5, idx[:, ::-1])
+    assert np.array_equal(o4, idx.T)
+    for o in range(8):
+        assert sorted(O.face_pts_order(o, 16)) == list(range(16))
+
+
+def test_face_points_map_to_faces():
+    fp = np.array([[-0.3, 0.6], [0.2, -0.9]])
+    fixed = {0: (2, -1), 1: (1, -1), 2: (0, 1), 3: (1, 1), 4: (0, -1), 5: (2, 1)}
+    for f, (d, val) in fixed.items():
+        for o in range(8):
+            x = O.points_on_face(f, o, fp)
+            assert np.all(x[:, d] == val)
+
+
+def test_normals_are_outward_area_weighted():
+    s = _solver(order=3, nelem=(2, 3, 4), lo=(-1, 0, 2), hi=(3, 1.5, 4))
+    h = np.array([4 / 2, 1.5 / 3, 2 / 4])
+    nqf = 25
+    N = s.normals(nqf)
+    d = np.arange(s.F) % 3
+    for k in range(3):
+        o = [i for i in range(3) if i != k]
+        expect = np.zeros(3); expect[k] = h[o[0]] * h[o[1]] / 4
+        assert np.allclose(N[d == k], expect, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("basis", ["LagrangeHex", "LegendreHex"])
+def test_inverse_mass_matrix(basis):
+    p = 3
+    s = _solver(order=p, basis=basis, nelem=(2, 2, 2), lo=(0, 0, 0), hi=(1, 2, 3))
+    iMM = s.imm(5)
+    # closed form: M = |J| (M1 x M1 x M1), M1 from a GL rule exact for degree 2p
+    x, w = O.gl_rule(p + 1)
+    V = np.array([O.basis_1d(basis, p, xi)[0] for xi in x])
+    M1 = V.T @ np.diag(w) @ V
+    J = (0.5 * 1.0 / 2) * (0.5 * 2.0 / 2) * (0.5 * 3.0 / 2)
+    M = J * np.kron(np.kron(M1, M1), M1)
+    assert np.allclose(iMM @ M, np.eye(64), atol=1e-11)
+    if basis == "LegendreHex":
+        n = np.arange(p + 1)
+        d1 = 2.0 / (2 * n + 1)
+        dd = np.kron(np.kron(d1, d1), d1) * J
+        assert np.allclose(np.diag(iMM), 1 / dd, rtol=1e-12)
+        assert np.max(np.abs(iMM - np.diag(np.diag(iMM)))) < 1e-10 * np.max(np.abs(iMM))
+
+
+@pytest.mark.parametrize("flux", ["LaxFriedrichs", "HLLC", "Roe"])
+@pytest.mark.parametrize("basis", ["LagrangeHex", "LegendreHex"])
+def test_free_stream_preservation(flux, basis):
+    s = _solver(order=2, basis=basis, flux=flux, nelem=(3, 2, 2), lo=(-1, -1, -1), hi=(1, 1, 1),
+                ic="Uniform", ic_data=(1.2, 0.3, -0.2, 0.5, 3.1))
+    s.init_state()
+    U = s.get_state()
+    R = s.residual(U)
+    assert np.max(np.abs(R)) < 1e-13
+
+
+@pytest.mark.parametrize("flux", ["LaxFriedrichs", "HLLC", "Roe"])
+def test_discrete_conservation(flux):
+    # sum over elements of the residual tested against phi = 1 vanishes on a
+    # periodic mesh (Lagrange: partition of unity over all basis functions)
+    s = _solver(order=2, basis="LagrangeHex", flux=flux, nelem=(4, 4, 4), lo=(-5, -5, -5), hi=(5, 5, 5),
+                ic="ExtrudedIsentropicVortex", ic_data=(1, 1, 1, 1, 5))
+    s.init_state()
+    U = s.get_state()
+    R = s.residual(U)
+    tot = R.sum(axis=(0, 1))
+    scale = np.abs(R).sum(axis=(0, 1))
+    assert np.all(np.abs(tot) < 1e-13 * np.max(scale))
+    assert np.max(np.abs(R)) > 1e-3
+
+
+def test_l2_projection_reproduces_polynomials():
+    # a uniform state is projected exactly; the L2 error against it is ~0
+    s = _solver(order=3, basis="LegendreHex", nelem=(2, 2, 2), ic="Uniform", ic_data=(1.0, 0.1, 0.2, 0.3, 2.5))
+    s.init_state()
+    U = s.get_state()
+    assert np.allclose(U[:, 0, :], [1.0, 0.1, 0.2, 0.3, 2.5] / np.array(1.0), rtol=1e-13)
+    assert np.max(np.abs(U[:, 1:, :])) < 1e-13
+    assert np.max(s.l2_error(0.0)) < 1e-14
+
+
+def test_reference_3d_vortex_regression(golden_dir):
+    """examples/euler/3D/isentropic_vortex/regression_data.txt:11-17 on a
+    regenerated periodic 4^3 unit cube, p=3, LagrangeHex, HLLC, RK4,
+    dt = 1e-3, end = 1.0 (int(1/1e-3) = 1000 steps)."""
+    g = json.load(open(os.path.join(golden_dir, "vortex3d_regression.json")))
+    s = _solver(order=3, basis="LagrangeHex", flux="HLLC", nelem=(4, 4, 4), lo=(0, 0, 0), hi=(1, 1, 1),
+                ic="IsentropicVortex", ic_data=(1, 1, 1, 1, 5))
+    s.init_state()
+    nsteps = int(g["end"] / g["timestep"])
+    assert nsteps == 1000
+    s.step("RK4", g["timestep"], nsteps)
+    e = s.l2_error(s.time)
+    ref = np.array([float(v) for v in g["l2_errors"]])
+    for v in (0, 1, 3, 4):
+        assert abs(e[v] - ref[v]) < 1e-10 * ref[v], (v, e[v], ref[v])
+    assert e[2] < 1e-14 and ref[2] < 1e-14     # rho v stays at round-off
