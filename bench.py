@@ -1,0 +1,175 @@
+"""Benchmark of the DG residual + RK4 path on one MI355X.
+
+Workload (the headline metric's configuration): 3D Euler, periodic cube
+[-5,5]^3 with 64^3 hexahedra, LegendreHex p=3 (64 DOF/element, 16.8M DOF x
+5 variables), Roe flux, classical RK4 at CFL 0.1 (dt from the initial
+state), z-extruded isentropic vortex (rho=1, u=v=1, w=0, p=1, beta=5) --
+synthetic, generated on the GPU.
+
+A step is one RK4 step = 4 fused stage launches.  W warm-up steps, then K
+timed steps bracketed by stream synchronisation; HIP events are recorded on
+the stream the stage kernels are launched on.
+
+value = DOF-updates per second per RK stage = E (p+1)^3 / t_stage.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dg-mpi_amd"))
+
+METRIC = "DOF-updates/s per RK stage, Euler p=3 64^3 elems; L2 err vs FP64 oracle"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
+# algorithmic HBM bytes per DOF per stage: classical RK4 register passes
+# (3 + 5 + 5 + 4) / 4 = 4.25 passes x 5 variables x 8 B
+ALG_BYTES_PER_DOF = 4.25 * 5 * 8
+P, NE = 3, 64
+
+
+def alg_flops_per_element(p):
+    """Sum-factorised DG operator count per element per stage (affine)."""
+    n, m, V = p + 1, p + 2, 5
+    S = n ** 3 * m + n ** 2 * m ** 2 + n * m ** 3
+    T = n ** 3 + n ** 2 * m + n * m ** 2
+    return 2 * (4 * V * S + 12 * V * T) + 60 * m ** 3 + 300 * m ** 2 + 4 * V * n ** 3
+
+
+def case_toml(ne, p=P, end=1.0):
+    import dg_amd
+    return dg_amd.case_toml(order=p, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne),
+                            lower=(-5., -5., -5.), upper=(5., 5., 5.), stepper="RK4", cfl=0.1, end=end,
+                            ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0))
+
+
+def measured_traffic():
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary,
+    if present (profiles/stage_kernel_p3_64cube_traffic.json)."""
+    f = os.path.join(ROOT, "profiles", "stage_kernel_p3_64cube_traffic.json")
+    if os.path.exists(f):
+        try:
+            return json.load(open(f)).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(budget_s=20.0):
+    """The CPU oracle (reference algorithm restated, dense per-element
+    GEMMs, OpenMP over elements) on a bounded sample of the workload: the
+    same case on 24^3 elements, p=3, Roe, RK4; RK4 steps until ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ne = 24
+    prm = O.make_params(order=P, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne), lo=(-5, -5, -5),
+                        hi=(5, 5, 5), gamma=1.4, R=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1, 1, 1, 1, 5))
+    s = O.Solver(prm)
+    s.init_state()
+    dt = 0.1 * (10.0 / ne) / s.max_wavespeed()
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        s.step("RK4", dt, 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 50:
+            break
+    dof = ne ** 3 * (P + 1) ** 3
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": dof * 4 * steps / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (dense reference formulation, OpenMP) on 24^3 elements of the same case "
+                      f"(p=3 LegendreHex, Roe, RK4), {steps} RK4 steps = {4 * steps} stages in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ne", type=int, default=NE, help=argparse.SUPPRESS)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path is measured at one GPU's real shape)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import dg_amd
+    from dg_amd import hip
+
+    assert hip.device_count() > 0, "bench.py needs a GPU"
+    ne = args.ne
+    s = dg_amd.Solver(case_toml(ne))
+    stream = hip.Stream()                 # default priority
+    s.set_stream(stream.handle)
+    s.init_state_from_fcn()
+    dt = s.timestep
+    E = ne ** 3
+    nb = (P + 1) ** 3
+    dof = E * nb
+
+    s.take_time_step(dt, args.warmup)
+    hip.synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    s.take_time_step(dt, args.steps)      # 4 stage launches per step, all on `stream`
+    ev1.record(stream)
+    ev1.synchronize()
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_ms(ev1)
+    ms_per_step = ms / args.steps
+    t_stage = ms_per_step / 4 * 1e-3
+    value = dof / t_stage
+
+    # accuracy of the timed run: L2 error vs the exact advected vortex
+    err = s.get_error()
+    finite = bool(np.all(np.isfinite(err)))
+
+    bytes_per_launch = ALG_BYTES_PER_DOF * dof
+    achieved = bytes_per_launch / t_stage / 1e9
+    flops_launch = alg_flops_per_element(P) * E
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (z-extruded isentropic vortex, generated on the GPU)",
+        "config": {"workload": f"euler3d_periodic_box_{ne}^3_p3_legendre_roe_rk4_cfl0.1",
+                   "elements": E, "order": P, "dof": dof, "variables": 5, "flux": "Roe",
+                   "basis": "LegendreHex", "stepper": "RK4", "dt": dt, "domain": "[-5,5]^3"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic()},
+        "kernel": {"name": "stage_kernel<3,Roe>", "avg_launch_ms": ms_per_step / 4,
+                   "launches": 4 * args.steps, "alg_bytes_per_launch": bytes_per_launch,
+                   "alg_flops_per_launch": flops_launch,
+                   "fp64_tflops": flops_launch / t_stage / 1e12,
+                   "fp64_frac": flops_launch / t_stage / 1e12 / FP64_PEAK_TFLOPS},
+        "l2_error_vs_exact": [float(x) for x in err],
+        "finite": finite,
+        "wall_s": wall,
+    }
+    s.close()
+    stream.destroy()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,81 @@
+// Host-visible declarations of the HIP kernels of the DG residual + RK path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace dg {
+
+constexpr int PMAX = 5;            // highest order with a compiled kernel
+constexpr int NVAR = 5;            // rho, rho u, rho v, rho w, rho E
+
+// Reference-element operators of one case (1D factors of the tensor bases).
+// Passed by value as kernel arguments (read through the scalar cache).
+struct KTables {
+    double B[PMAX + 2][PMAX + 1];   // phi_i(x_a), a: (p+2)-point GL rule
+    double D[PMAX + 2][PMAX + 1];   // phi_i'(x_a)
+    double phim[PMAX + 1];          // phi_i(-1)
+    double phip[PMAX + 1];          // phi_i(+1)
+    double w[PMAX + 2];             // GL weights of the (p+2)-point rule
+    double minv[PMAX + 1][PMAX + 1];// inverse of the 1D mass matrix
+    double BQ[2 * PMAX + 2][PMAX + 1];  // phi_i at the (2p+2)-point rule
+    double xq[2 * PMAX + 2];        // (2p+2)-point GL nodes
+    double wq[2 * PMAX + 2];        // (2p+2)-point GL weights
+    int minv_diag;                  // 1: minv is diagonal (Legendre)
+};
+
+// Structured periodic box and physics constants.
+struct KGeom {
+    int ne[3];                      // elements per direction
+    unsigned E;                     // number of elements
+    double lo[3], h[3];             // box corner, element widths
+    double period[3];               // box lengths
+    double gvol[3];                 // (d xi_d / d x_d) |J|: contravariant flux scale
+    double farea[3];                // |N| of faces normal to d: h_t1 h_t2 / 4
+    double jdet, jinv;              // |J| = hx hy hz / 8 and its inverse
+    double gamma, R;
+    int xcd_remap;                  // 1: map workgroups to elements XCD-contiguously
+};
+
+// Initial condition / exact solution selector and data.
+struct KIC {
+    int ic;                         // enum dg::IC
+    double data[10];
+};
+
+// Epilogue of the fused stage kernel.
+enum StageMode {
+    MODE_RESIDUAL = 0,   // R = volume - face (Solver::get_residual)
+    MODE_RK_FIRST = 1,   // Uout = U0 + alpha dU; acc = dU          (RK4 stage 1)
+    MODE_RK_MID = 2,     // Uout = U0 + alpha dU; acc += 2 dU       (RK4 stages 2, 3)
+    MODE_RK_LAST = 3,    // U0 += (acc + dU) / 6                    (RK4 stage 4)
+    MODE_FE = 4          // Uout = U0 + dU                          (forward Euler)
+};
+
+struct StageArgs {
+    const double* Ust;   // state the residual is evaluated at (neighbours read it)
+    const double* U0;    // base state of the RK update
+    double* Uout;        // next stage state
+    double* acc;         // RK accumulator
+    double* R;           // residual output (MODE_RESIDUAL)
+    double dt;           // dU = dt M^-1 R
+    double alpha;        // stage coefficient
+    int mode;
+};
+
+// Launchers; P in 1..PMAX, flux 0 LLF / 1 HLLC / 2 Roe.  Return the launch
+// status (hipGetLastError after the launch).
+hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables& t, const KGeom& g,
+                        hipStream_t st);
+hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables& t,
+                           const KGeom& g, hipStream_t st);
+hipError_t launch_init_state(int P, double time, double* U, const KTables& t, const KGeom& g,
+                             const KIC& ic, hipStream_t st);
+// mode 0: per-element squared L2 error vs exact solution at time t -> out[E][5]
+// mode 1: per-element integrals (rho, rho u, rho v, rho w, rho E, KE) -> out[E][6]
+hipError_t launch_eval(int P, int mode, double time, const double* U, double* out,
+                       const KTables& t, const KGeom& g, const KIC& ic, hipStream_t st);
+// per-element max(|u| + c) at the volume points -> out[E]
+hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables& t, const KGeom& g,
+                            hipStream_t st);
+
+}  // namespace dg

@@ -1,0 +1,423 @@
+// C ABI of the DG residual + RK path (include/dg_amd.h).
+#include "dg_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "dg_config.hpp"
+#include "dg_kernels.hpp"
+#include "dg_tables.hpp"
+
+struct dg_solver {
+    dg::CaseConfig cfg;
+    dg::KTables tab;
+    dg::KGeom geo;
+    dg::KIC ic;
+    int P = 0;
+    size_t ncoef = 0;                  // E * nb * 5
+    double dt = 0.0;                   // case time step (0 until known)
+    double time = 0.0;
+    int device = 0;
+    hipStream_t stream = nullptr;      // stream the handle launches on
+    hipStream_t own_stream = nullptr;  // stream created by dg_create
+    double* U0 = nullptr;              // solution (Solver::Uc)
+    double* Ua = nullptr;              // stage states
+    double* Ub = nullptr;
+    double* acc = nullptr;             // RK accumulator
+    double* work = nullptr;            // per-element reduction output
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        std::ostringstream o;
+        o << what << ": " << hipGetErrorString(e);
+        throw DeviceError(o.str());
+    }
+}
+
+template <class F>
+int guarded(dg_solver* s, F&& f) {
+    try {
+        f();
+        if (s) s->err.clear();
+        return DG_OK;
+    } catch (const dg::InputError& e) {
+        if (s) s->err = e.what(); else g_create_error = e.what();
+        return DG_ERR_INPUT;
+    } catch (const dg::UnsupportedError& e) {
+        if (s) s->err = e.what(); else g_create_error = e.what();
+        return DG_ERR_UNSUPPORTED;
+    } catch (const DeviceError& e) {
+        if (s) s->err = e.what(); else g_create_error = e.what();
+        return DG_ERR_DEVICE;
+    } catch (const std::invalid_argument& e) {
+        if (s) s->err = e.what(); else g_create_error = e.what();
+        return DG_ERR_ARGUMENT;
+    } catch (const std::domain_error& e) {
+        if (s) s->err = e.what(); else g_create_error = e.what();
+        return DG_ERR_NUMERICS;
+    } catch (const std::bad_alloc&) {
+        if (s) s->err = "host out of memory"; else g_create_error = "host out of memory";
+        return DG_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        if (s) s->err = e.what(); else g_create_error = e.what();
+        return DG_ERR_INPUT;
+    }
+}
+
+void need(const void* p, const char* what) {
+    if (!p) throw std::invalid_argument(std::string(what) + " is NULL");
+}
+
+hipStream_t pick_stream(dg_solver* s, void* stream) {
+    return stream ? (hipStream_t)stream : s->stream;
+}
+
+void set_device(const dg_solver* s) { hip_check(hipSetDevice(s->device), "hipSetDevice"); }
+
+// Neumaier-compensated sum of x[i*stride + off]
+double csum(const std::vector<double>& x, size_t n, size_t stride, size_t off) {
+    double sum = 0.0, c = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double v = x[i * stride + off];
+        const double t = sum + v;
+        if (std::fabs(sum) >= std::fabs(v)) c += (sum - t) + v;
+        else c += (v - t) + sum;
+        sum = t;
+    }
+    return sum + c;
+}
+
+double max_wavespeed(dg_solver* s, const double* U) {
+    hip_check(dg::launch_wavespeed(s->P, U, s->work, s->tab, s->geo, s->stream), "wavespeed kernel");
+    std::vector<double> h(s->geo.E);
+    hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
+              "hipMemcpy");
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    double m = 0.0;
+    for (double v : h) {
+        if (!std::isfinite(v)) throw std::domain_error("non-finite state (wave speed)");
+        if (v > m) m = v;
+    }
+    return m;
+}
+
+// one time step of the configured stepper on U0 (stream-ordered)
+void step_once(dg_solver* s, double dt) {
+    using namespace dg;
+    StageArgs a{};
+    a.dt = dt;
+    hipStream_t st = s->stream;
+    const int flux = (int)s->cfg.flux;
+    if (s->cfg.stepper == Stepper::FE) {
+        // the stage reads neighbours of U0 while other elements update, so
+        // the new state goes to Ua and is copied back (the state pointer
+        // handed out by dg_state_device_ptr stays valid)
+        a.Ust = s->U0; a.U0 = s->U0; a.Uout = s->Ua; a.mode = MODE_FE;
+        hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+        hip_check(hipMemcpyAsync(s->U0, s->Ua, s->ncoef * sizeof(double), hipMemcpyDeviceToDevice, st),
+                  "hipMemcpyAsync");
+        s->time += dt;
+        return;
+    }
+    // RK4 (src/numerics/timestepping/stepper.cpp:71-142), fused stages
+    a.U0 = s->U0; a.acc = s->acc;
+    a.Ust = s->U0; a.Uout = s->Ua; a.alpha = 0.5; a.mode = MODE_RK_FIRST;
+    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+    s->time += dt / 2.0;
+    a.Ust = s->Ua; a.Uout = s->Ub; a.alpha = 0.5; a.mode = MODE_RK_MID;
+    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+    a.Ust = s->Ub; a.Uout = s->Ua; a.alpha = 1.0; a.mode = MODE_RK_MID;
+    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+    s->time += dt / 2.0;
+    a.Ust = s->Ua; a.Uout = s->U0; a.mode = MODE_RK_LAST;
+    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+}
+
+void free_all(dg_solver* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work})
+        if (p) (void)hipFree(p);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    s->U0 = s->Ua = s->Ub = s->acc = s->work = nullptr;
+    s->stream = s->own_stream = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_create(const char* toml_text, dg_solver** out) {
+    if (out) *out = nullptr;
+    dg_solver* s = nullptr;
+    int rc = guarded(nullptr, [&] {
+        need(toml_text, "toml_text");
+        need(out, "out");
+        dg::CaseConfig cfg = dg::parse_case(toml_text);
+        s = new dg_solver();
+        s->cfg = cfg;
+        s->P = cfg.order;
+        s->tab = dg::make_tables(cfg);
+        s->geo = dg::make_geometry(cfg);
+        s->ic = dg::make_ic(cfg);
+        s->dt = cfg.timestep;
+        const size_t nb = (size_t)(s->P + 1) * (s->P + 1) * (s->P + 1);
+        s->ncoef = (size_t)s->geo.E * nb * dg::NVAR;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw DeviceError("no HIP device available");
+        hip_check(hipGetDevice(&s->device), "hipGetDevice");
+        hip_check(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+        s->stream = s->own_stream;
+        const size_t bytes = s->ncoef * sizeof(double);
+        hip_check(hipMalloc(&s->U0, bytes), "hipMalloc(U0)");
+        hip_check(hipMalloc(&s->Ua, bytes), "hipMalloc(Ua)");
+        hip_check(hipMalloc(&s->Ub, bytes), "hipMalloc(Ub)");
+        hip_check(hipMalloc(&s->acc, bytes), "hipMalloc(acc)");
+        hip_check(hipMalloc(&s->work, (size_t)s->geo.E * 6 * sizeof(double)), "hipMalloc(work)");
+        hip_check(hipMemsetAsync(s->U0, 0, bytes, s->stream), "hipMemset");
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    });
+    if (rc != DG_OK) {
+        free_all(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return DG_OK;
+}
+
+int dg_destroy(dg_solver* s) {
+    if (!s) return DG_OK;
+    free_all(s);
+    delete s;
+    return DG_OK;
+}
+
+const char* dg_last_error(const dg_solver* s) {
+    return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int dg_get_info(const dg_solver* s, dg_info* info) {
+    return guarded(const_cast<dg_solver*>(s), [&] {
+        need(s, "solver");
+        need(info, "info");
+        std::memset(info, 0, sizeof(*info));
+        const auto& c = s->cfg;
+        info->num_elems = s->geo.E;
+        info->order = c.order;
+        info->num_basis = (c.order + 1) * (c.order + 1) * (c.order + 1);
+        info->num_state_vars = dg::NVAR;
+        for (int d = 0; d < 3; ++d) {
+            info->nelem[d] = c.nelem[d];
+            info->lower[d] = c.lower[d];
+            info->upper[d] = c.upper[d];
+        }
+        info->gamma = c.gamma;
+        info->gas_constant = c.gas_constant;
+        info->timestep = s->dt;
+        info->end_time = c.end_time;
+        info->time = s->time;
+        info->num_iterations = s->dt > 0 ? (int)(c.end_time / s->dt) : 0;
+        info->stepper = (int)c.stepper;
+        info->flux = (int)c.flux;
+        info->basis = (int)c.basis;
+        info->ic = (int)c.ic;
+    });
+}
+
+int dg_init_state(dg_solver* s) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        set_device(s);
+        s->time = 0.0;
+        hip_check(dg::launch_init_state(s->P, s->time, s->U0, s->tab, s->geo, s->ic, s->stream), "init kernel");
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        if (s->cfg.timestep <= 0.0) {
+            // CFL rule (new): dt = cfl * min(h) / max(|u| + c) of the initial state
+            const double amax = max_wavespeed(s, s->U0);
+            if (!(amax > 0.0)) throw std::domain_error("zero wave speed: cannot derive dt from Stepper.cfl");
+            const double hmin = std::fmin(s->geo.h[0], std::fmin(s->geo.h[1], s->geo.h[2]));
+            s->dt = s->cfg.cfl * hmin / amax;
+        }
+    });
+}
+
+int dg_set_state(dg_solver* s, const double* host, int layout) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(host, "host");
+        set_device(s);
+        if (layout == DG_LAYOUT_RIGHT) {
+            hip_check(hipMemcpy(s->U0, host, s->ncoef * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        } else if (layout == DG_LAYOUT_LEFT) {
+            const size_t E = s->geo.E, nb = s->ncoef / (E * dg::NVAR);
+            std::vector<double> t(s->ncoef);
+            for (size_t v = 0; v < dg::NVAR; ++v)
+                for (size_t b = 0; b < nb; ++b)
+                    for (size_t e = 0; e < E; ++e) t[(e * nb + b) * dg::NVAR + v] = host[(v * nb + b) * E + e];
+            hip_check(hipMemcpy(s->U0, t.data(), s->ncoef * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        } else {
+            throw std::invalid_argument("unknown layout");
+        }
+    });
+}
+
+int dg_get_state(const dg_solver* cs, double* host, int layout) {
+    dg_solver* s = const_cast<dg_solver*>(cs);
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(host, "host");
+        set_device(s);
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        if (layout == DG_LAYOUT_RIGHT) {
+            hip_check(hipMemcpy(host, s->U0, s->ncoef * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+        } else if (layout == DG_LAYOUT_LEFT) {
+            const size_t E = s->geo.E, nb = s->ncoef / (E * dg::NVAR);
+            std::vector<double> t(s->ncoef);
+            hip_check(hipMemcpy(t.data(), s->U0, s->ncoef * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+            for (size_t v = 0; v < dg::NVAR; ++v)
+                for (size_t b = 0; b < nb; ++b)
+                    for (size_t e = 0; e < E; ++e) host[(v * nb + b) * E + e] = t[(e * nb + b) * dg::NVAR + v];
+        } else {
+            throw std::invalid_argument("unknown layout");
+        }
+    });
+}
+
+int dg_state_device_ptr(dg_solver* s, double** dev) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(dev, "dev");
+        *dev = s->U0;
+    });
+}
+
+int dg_residual(dg_solver* s, const double* U_dev, double* R_dev, void* stream) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(U_dev, "U_dev");
+        need(R_dev, "R_dev");
+        if ((const void*)U_dev == (const void*)R_dev) throw std::invalid_argument("U_dev and R_dev alias");
+        set_device(s);
+        dg::StageArgs a{};
+        a.Ust = U_dev;
+        a.R = R_dev;
+        a.mode = dg::MODE_RESIDUAL;
+        hip_check(dg::launch_stage(s->P, (int)s->cfg.flux, a, s->tab, s->geo, pick_stream(s, stream)),
+                  "stage kernel");
+    });
+}
+
+int dg_apply_inv_mass(dg_solver* s, double dt, const double* R_dev, double* dU_dev, void* stream) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(R_dev, "R_dev");
+        need(dU_dev, "dU_dev");
+        set_device(s);
+        hip_check(dg::launch_inv_mass(s->P, dt, R_dev, dU_dev, s->tab, s->geo, pick_stream(s, stream)),
+                  "inverse mass kernel");
+    });
+}
+
+int dg_step(dg_solver* s, double dt, int nsteps) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
+        set_device(s);
+        if (dt <= 0.0) dt = s->dt;
+        if (!(dt > 0.0)) throw std::invalid_argument("no time step: give dt, Stepper.timestep or call dg_init_state with Stepper.cfl");
+        for (int i = 0; i < nsteps; ++i) step_once(s, dt);
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    });
+}
+
+int dg_solve(dg_solver* s) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        if (!(s->dt > 0.0)) throw std::invalid_argument("no time step: call dg_init_state first with Stepper.cfl");
+        set_device(s);
+        const int n = (int)(s->cfg.end_time / s->dt);     // src/solver/base.cpp:383
+        for (int i = 0; i < n; ++i) step_once(s, s->dt);
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    });
+}
+
+int dg_l2_error(dg_solver* s, double t, double err[5]) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(err, "err");
+        set_device(s);
+        hip_check(dg::launch_eval(s->P, 0, t, s->U0, s->work, s->tab, s->geo, s->ic, s->stream), "eval kernel");
+        std::vector<double> h((size_t)s->geo.E * 6);
+        hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
+                  "hipMemcpy");
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        // total volume: sum of element volumes on the (p+2)^3 rule
+        // (src/mesh/tools.cpp:27-47, src/post/main.cpp:49-54)
+        double wsum = 0.0;
+        for (int a = 0; a < s->P + 2; ++a)
+            for (int b = 0; b < s->P + 2; ++b)
+                for (int c = 0; c < s->P + 2; ++c) wsum += s->tab.w[a] * s->tab.w[b] * s->tab.w[c];
+        const double vol = (double)s->geo.E * (wsum * s->geo.jdet);
+        for (int v = 0; v < dg::NVAR; ++v) err[v] = std::pow(csum(h, s->geo.E, 6, v) / vol, 1. / 2);
+    });
+}
+
+int dg_integrals(dg_solver* s, double out[6]) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(out, "out");
+        set_device(s);
+        hip_check(dg::launch_eval(s->P, 1, s->time, s->U0, s->work, s->tab, s->geo, s->ic, s->stream), "eval kernel");
+        std::vector<double> h((size_t)s->geo.E * 6);
+        hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
+                  "hipMemcpy");
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        for (int v = 0; v < 6; ++v) out[v] = csum(h, s->geo.E, 6, v);
+    });
+}
+
+int dg_max_wavespeed(dg_solver* s, double* amax) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        need(amax, "amax");
+        set_device(s);
+        *amax = max_wavespeed(s, s->U0);
+    });
+}
+
+int dg_set_time(dg_solver* s, double t) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        s->time = t;
+    });
+}
+
+int dg_set_stream(dg_solver* s, void* stream) {
+    return guarded(s, [&] {
+        need(s, "solver");
+        set_device(s);
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        s->stream = stream ? (hipStream_t)stream : s->own_stream;
+    });
+}
+
+}  // extern "C"

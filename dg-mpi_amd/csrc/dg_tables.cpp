@@ -1,0 +1,198 @@
+// Reference-element tables of the tensor-product hex bases.
+#include "dg_tables.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace dg {
+
+namespace {
+using q128 = __float128;
+
+q128 qabs(q128 x) { return x < 0 ? -x : x; }
+
+// P_n(z) and P_n'(z) by the three-term recurrence, binary128.
+void legendre_q(int n, q128 z, q128& pn, q128& dpn) {
+    q128 p0 = 1, p1 = z;
+    for (int k = 2; k <= n; ++k) {
+        q128 p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    pn = n == 0 ? q128(1) : p1;
+    dpn = n * (z * p1 - p0) / (z * z - 1);
+}
+}  // namespace
+
+void gl_rule(int n, double* x, double* w) {
+    if (n < 1) throw std::invalid_argument("gl_rule: n < 1");
+    if (n == 1) {
+        x[0] = 0.0;
+        w[0] = 2.0;
+        return;
+    }
+    for (int i = 0; i < n / 2; ++i) {
+        // Newton from the Tricomi-style initial guess; converged roots are
+        // refined in binary128 then rounded once to double
+        q128 z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        q128 pn, dp;
+        for (int it = 0; it < 100; ++it) {
+            legendre_q(n, z, pn, dp);
+            q128 dz = pn / dp;
+            z -= dz;
+            if (qabs(dz) < q128(1e-33)) break;
+        }
+        legendre_q(n, z, pn, dp);
+        q128 wq = 2 / ((1 - z * z) * dp * dp);
+        x[n - 1 - i] = (double)z;
+        x[i] = -(double)z;
+        w[n - 1 - i] = (double)wq;
+        w[i] = (double)wq;
+    }
+    if (n % 2 == 1) {
+        q128 pn, dp;
+        legendre_q(n, 0, pn, dp);
+        x[n / 2] = 0.0;
+        w[n / 2] = (double)(2 / (dp * dp));
+    }
+}
+
+void basis_1d(Basis b, int p, double x, double* val, double* grad) {
+    if (b == Basis::LagrangeHex) {
+        double xn[16];
+        const int nn = p + 1;
+        const double dx = (1. - -1.) / ((double)nn - 1.);
+        for (int i = 0; i < nn; i++) xn[i] = -1. + (double)i * dx;
+        for (int i = 0; i < nn; i++) {
+            double v = 1.;
+            for (int j = 0; j < nn; j++)
+                if (j != i) v *= (x - xn[j]) / (xn[i] - xn[j]);
+            val[i] = v;
+            double g = 0.;
+            for (int j = 0; j < nn; j++) {
+                if (j == i) continue;
+                double prod = 1.;
+                for (int k = 0; k < nn; k++) {
+                    if (k == i) continue;
+                    prod *= (k == j) ? 1. / (xn[i] - xn[k]) : (x - xn[k]) / (xn[i] - xn[k]);
+                }
+                g += prod;
+            }
+            grad[i] = g;
+        }
+        return;
+    }
+    // unnormalised Legendre polynomials and derivatives, p <= 7
+    const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x, x6 = x5 * x, x7 = x6 * x;
+    const double v[8] = {1., x, 0.5 * (3. * x2 - 1.), 0.5 * (5. * x3 - 3. * x),
+                         0.125 * (35. * x4 - 30. * x2 + 3.), 0.125 * (63. * x5 - 70. * x3 + 15. * x),
+                         0.0625 * (231. * x6 - 315. * x4 + 105. * x2 - 5.),
+                         0.0625 * (429. * x7 - 693. * x5 + 315. * x3 - 35. * x)};
+    const double g[8] = {0., 1., 3. * x, 0.5 * (15. * x2 - 3.), 0.125 * (140. * x3 - 60. * x),
+                         0.125 * (315. * x4 - 210. * x2 + 15.),
+                         0.0625 * (1386. * x5 - 1260. * x3 + 210. * x),
+                         0.0625 * (3003. * x6 - 3465. * x4 + 945. * x2 - 35.)};
+    for (int i = 0; i <= p; ++i) {
+        val[i] = v[i];
+        grad[i] = g[i];
+    }
+}
+
+KTables make_tables(const CaseConfig& c) {
+    const int p = c.order, n = p + 1, m = p + 2, q = 2 * p + 2;
+    if (p < 1 || p > PMAX) throw UnsupportedError("order p = " + std::to_string(p) + " has no GPU kernel (1.." +
+                                                  std::to_string(PMAX) + ")");
+    KTables t;
+    std::memset(&t, 0, sizeof(t));
+    double x[32], w[32], v[16], g[16];
+    // (p+2)-point rule: volume and face quadrature (quadrature/tools.cpp:7-24,
+    // quadrilateral.cpp:100-105)
+    gl_rule(m, x, w);
+    for (int a = 0; a < m; ++a) {
+        basis_1d(c.basis, p, x[a], v, g);
+        for (int i = 0; i < n; ++i) {
+            t.B[a][i] = v[i];
+            t.D[a][i] = g[i];
+        }
+        t.w[a] = w[a];
+    }
+    basis_1d(c.basis, p, -1.0, v, g);
+    for (int i = 0; i < n; ++i) t.phim[i] = v[i];
+    basis_1d(c.basis, p, 1.0, v, g);
+    for (int i = 0; i < n; ++i) t.phip[i] = v[i];
+    // (2p+2)-point rule: mass matrix, IC projection, L2 error
+    gl_rule(q, x, w);
+    for (int a = 0; a < q; ++a) {
+        basis_1d(c.basis, p, x[a], v, g);
+        for (int i = 0; i < n; ++i) t.BQ[a][i] = v[i];
+        t.xq[a] = x[a];
+        t.wq[a] = w[a];
+    }
+    // 1D mass matrix on the (2p+2)-point rule (exact for degree 2p) and its
+    // inverse by Gauss-Jordan elimination in binary128
+    q128 Mq[8][16];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < 2 * n; ++j) Mq[i][j] = 0;
+    for (int a = 0; a < q; ++a)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) Mq[i][j] += (q128)t.BQ[a][i] * (q128)t.BQ[a][j] * (q128)w[a];
+    for (int i = 0; i < n; ++i) Mq[i][n + i] = 1;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (qabs(Mq[r][col]) > qabs(Mq[piv][col])) piv = r;
+        if (piv != col)
+            for (int j = 0; j < 2 * n; ++j) { q128 tt = Mq[col][j]; Mq[col][j] = Mq[piv][j]; Mq[piv][j] = tt; }
+        q128 d = Mq[col][col];
+        for (int j = 0; j < 2 * n; ++j) Mq[col][j] /= d;
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            q128 f = Mq[r][col];
+            for (int j = 0; j < 2 * n; ++j) Mq[r][j] -= f * Mq[col][j];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) t.minv[i][j] = (double)Mq[i][n + j];
+    t.minv_diag = c.basis == Basis::LegendreHex ? 1 : 0;
+    if (t.minv_diag)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                if (i != j) t.minv[i][j] = 0.0;
+    return t;
+}
+
+KGeom make_geometry(const CaseConfig& c) {
+    KGeom g;
+    std::memset(&g, 0, sizeof(g));
+    long E = 1;
+    for (int d = 0; d < 3; ++d) {
+        g.ne[d] = c.nelem[d];
+        g.lo[d] = c.lower[d];
+        g.period[d] = c.upper[d] - c.lower[d];
+        g.h[d] = g.period[d] / c.nelem[d];
+        E *= c.nelem[d];
+    }
+    g.E = (unsigned)E;
+    g.jdet = (0.5 * g.h[0]) * (0.5 * g.h[1]) * (0.5 * g.h[2]);
+    g.jinv = 1.0 / g.jdet;
+    for (int d = 0; d < 3; ++d) {
+        g.gvol[d] = (2.0 / g.h[d]) * g.jdet;
+        const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+        g.farea[d] = (0.5 * g.h[t1]) * (0.5 * g.h[t2]);
+    }
+    g.gamma = c.gamma;
+    g.R = c.gas_constant;
+    g.xcd_remap = (E % 8 == 0 && E >= 8 * 64) ? 1 : 0;
+    return g;
+}
+
+KIC make_ic(const CaseConfig& c) {
+    KIC k;
+    std::memset(&k, 0, sizeof(k));
+    k.ic = (int)c.ic;
+    for (size_t i = 0; i < c.ic_data.size() && i < 10; ++i) k.data[i] = c.ic_data[i];
+    return k;
+}
+
+}  // namespace dg

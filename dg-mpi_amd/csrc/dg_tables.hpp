@@ -1,0 +1,29 @@
+// Reference-element tables of the tensor-product hex bases.
+#pragma once
+
+#include <vector>
+
+#include "dg_config.hpp"
+#include "dg_kernels.hpp"
+
+namespace dg {
+
+// n-point Gauss-Legendre rule on [-1,1], ascending (the reference's
+// tabulated rules, src/numerics/nodes.cpp:97-500 and
+// src/numerics/quadrature/segment.cpp:7-411, to the nearest double).
+void gl_rule(int n, double* x, double* w);
+
+// 1D factors phi_i(x), phi_i'(x) of LagrangeHex (equidistant nodes,
+// src/numerics/basis/tools.cpp:9-25,115-149) or LegendreHex (unnormalised
+// P_n, src/numerics/basis/tools.cpp:298-366).
+void basis_1d(Basis b, int p, double x, double* val, double* grad);
+
+// Kernel tables for order p (p <= PMAX).
+KTables make_tables(const CaseConfig& c);
+
+// Structured-box geometry and physics constants.
+KGeom make_geometry(const CaseConfig& c);
+
+KIC make_ic(const CaseConfig& c);
+
+}  // namespace dg

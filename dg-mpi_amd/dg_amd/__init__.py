@@ -1,0 +1,247 @@
+"""Python binding of the MI355X DG residual + RK path (libdg_amd.so).
+
+Mirrors the reference's solver surface (src/solver/base.h:16-90,
+src/numerics/timestepping/stepper.h:14-90, src/post/main.cpp:30-201):
+
+    s = Solver(toml_text)          # Solver<3>(input, mesh, ...) + precompute
+    s.init_state_from_fcn()        # L2 projection of the initial condition
+    s.get_residual(U_ptr, R_ptr)   # R = volume - face (device buffers)
+    s.take_time_step()             # one FE / RK4 step
+    s.solve()                      # (int)(end / dt) steps
+    s.get_error()                  # volume-normalised L2 error per variable
+
+The compute path is the HIP library only: importing fails when it is not
+built, and creating a solver fails without a GPU.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdg_amd.so")
+
+DG_LAYOUT_RIGHT = 0
+DG_LAYOUT_LEFT = 1
+STATUS = {0: "DG_OK", 1: "DG_ERR_INPUT", 2: "DG_ERR_DEVICE", 3: "DG_ERR_ARGUMENT",
+          4: "DG_ERR_UNSUPPORTED", 5: "DG_ERR_NUMERICS"}
+FLUX_NAMES = ["LaxFriedrichs", "HLLC", "Roe"]
+BASIS_NAMES = ["LagrangeHex", "LegendreHex"]
+STEPPER_NAMES = ["FE", "RK4"]
+
+# every symbol include/dg_amd.h declares
+EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_get_info", "dg_init_state",
+           "dg_set_state", "dg_get_state", "dg_state_device_ptr", "dg_residual",
+           "dg_apply_inv_mass", "dg_step", "dg_solve", "dg_l2_error", "dg_integrals",
+           "dg_max_wavespeed", "dg_set_time", "dg_set_stream"]
+
+
+class DGError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Info(C.Structure):
+    _fields_ = [("num_elems", C.c_long), ("order", C.c_int), ("num_basis", C.c_int),
+                ("num_state_vars", C.c_int), ("nelem", C.c_int * 3),
+                ("lower", C.c_double * 3), ("upper", C.c_double * 3),
+                ("gamma", C.c_double), ("gas_constant", C.c_double),
+                ("timestep", C.c_double), ("end_time", C.c_double), ("time", C.c_double),
+                ("num_iterations", C.c_int), ("stepper", C.c_int), ("flux", C.c_int),
+                ("basis", C.c_int), ("ic", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libdg_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                              "(make -C dg-mpi_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        sig = {
+            "dg_create": [C.c_char_p, C.POINTER(vp)],
+            "dg_destroy": [vp],
+            "dg_get_info": [vp, C.POINTER(Info)],
+            "dg_init_state": [vp],
+            "dg_set_state": [vp, dp, C.c_int],
+            "dg_get_state": [vp, dp, C.c_int],
+            "dg_state_device_ptr": [vp, C.POINTER(dp)],
+            "dg_residual": [vp, vp, vp, vp],
+            "dg_apply_inv_mass": [vp, C.c_double, vp, vp, vp],
+            "dg_step": [vp, C.c_double, C.c_int],
+            "dg_solve": [vp],
+            "dg_l2_error": [vp, C.c_double, dp],
+            "dg_integrals": [vp, dp],
+            "dg_max_wavespeed": [vp, dp],
+            "dg_set_time": [vp, C.c_double],
+            "dg_set_stream": [vp, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.dg_last_error.argtypes = [vp]
+        L.dg_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One DG case on the current HIP device."""
+
+    def __init__(self, toml_text):
+        h = C.c_void_p()
+        rc = lib().dg_create(toml_text.encode(), C.byref(h))
+        if rc != 0:
+            raise DGError(rc, lib().dg_last_error(None).decode())
+        self._h = h
+        self.info = self._info()
+
+    # -- plumbing --
+    def _check(self, rc):
+        if rc != 0:
+            raise DGError(rc, lib().dg_last_error(self._h).decode())
+
+    def _info(self):
+        i = Info()
+        self._check(lib().dg_get_info(self._h, C.byref(i)))
+        return i
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def shape(self):
+        i = self.info
+        return (i.num_elems, i.num_basis, i.num_state_vars)
+
+    @property
+    def time(self):
+        return self._info().time
+
+    @property
+    def timestep(self):
+        return self._info().timestep
+
+    @property
+    def num_iterations(self):
+        return self._info().num_iterations
+
+    # -- reference surface --
+    def init_state_from_fcn(self):
+        self._check(lib().dg_init_state(self._h))
+        self.info = self._info()
+
+    def get_state(self, layout=DG_LAYOUT_RIGHT):
+        i = self.info
+        U = np.zeros(i.num_elems * i.num_basis * i.num_state_vars)
+        self._check(lib().dg_get_state(self._h, _ptr(U), layout))
+        return U.reshape(self.shape) if layout == DG_LAYOUT_RIGHT else U.reshape(
+            (i.num_state_vars, i.num_basis, i.num_elems))
+
+    def set_state(self, U, layout=DG_LAYOUT_RIGHT):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.size != int(np.prod(self.shape)):
+            raise ValueError("state has the wrong size")
+        self._check(lib().dg_set_state(self._h, _ptr(U), layout))
+
+    def state_device_ptr(self):
+        p = C.POINTER(C.c_double)()
+        self._check(lib().dg_state_device_ptr(self._h, C.byref(p)))
+        return C.cast(p, C.c_void_p).value
+
+    def get_residual(self, U_dev, R_dev, stream=None):
+        """R = volume - face at U (device pointers, [E][nb][5] FP64)."""
+        self._check(lib().dg_residual(self._h, C.c_void_p(U_dev), C.c_void_p(R_dev),
+                                      C.c_void_p(stream) if stream else None))
+
+    def mult_inv_mass_matrix(self, dt, R_dev, dU_dev, stream=None):
+        self._check(lib().dg_apply_inv_mass(self._h, dt, C.c_void_p(R_dev), C.c_void_p(dU_dev),
+                                            C.c_void_p(stream) if stream else None))
+
+    def take_time_step(self, dt=0.0, nsteps=1):
+        self._check(lib().dg_step(self._h, dt, nsteps))
+
+    def solve(self):
+        self._check(lib().dg_solve(self._h))
+
+    def get_error(self, t=None):
+        e = np.zeros(5)
+        self._check(lib().dg_l2_error(self._h, self.time if t is None else t, _ptr(e)))
+        return e
+
+    def integrals(self):
+        o = np.zeros(6)
+        self._check(lib().dg_integrals(self._h, _ptr(o)))
+        return o
+
+    def max_wavespeed(self):
+        a = np.zeros(1)
+        self._check(lib().dg_max_wavespeed(self._h, _ptr(a)))
+        return float(a[0])
+
+    def set_time(self, t):
+        self._check(lib().dg_set_time(self._h, t))
+
+    def set_stream(self, stream):
+        """Launch on a caller-owned HIP stream handle (int), None = own."""
+        self._check(lib().dg_set_stream(self._h, C.c_void_p(stream) if stream else None))
+
+
+def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(-5., -5., -5.),
+              upper=(5., 5., 5.), stepper="RK4", timestep=None, cfl=None, end=1.0, gamma=1.4,
+              gas_constant=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0)):
+    """TOML text of a periodic-box case (keys of the reference input decks)."""
+    st = f"timestep = {timestep!r}" if timestep is not None else f"cfl = {cfl!r}"
+    arr = lambda v: "[" + ", ".join(repr(float(x)) for x in v) + "]"
+    return f"""[Mesh]
+type = "PeriodicBox"
+nelem = [{nelem[0]}, {nelem[1]}, {nelem[2]}]
+lower = {arr(lower)}
+upper = {arr(upper)}
+
+[Numerics]
+order = {order}
+basis = "{basis}"
+
+[Stepper]
+type = "{stepper}"
+{st}
+end = {float(end)!r}
+
+[Physics]
+name = "Euler"
+dim = 3
+gamma = {gamma!r}
+GasConstant = {gas_constant!r}
+convective_flux_fcn = "{flux}"
+
+[InitialCondition]
+name = "{ic}"
+data = {arr(ic_data)}
+"""

@@ -1,0 +1,135 @@
+"""Minimal HIP runtime plumbing (device buffers, streams, events) over the
+same libamdhip64 that libdg_amd.so links, so that pointers and stream handles
+passed through the C ABI belong to one runtime.  (PyTorch wheels bundle a
+second HIP runtime with the same soname; mixing the two in one process is
+avoided by using this module instead.)"""
+import ctypes as C
+
+import numpy as np
+
+_h = None
+
+
+def rt():
+    global _h
+    if _h is None:
+        from . import lib
+        lib()                                  # loads libamdhip64.so.7 as libdg_amd.so's dependency
+        H = C.CDLL("libamdhip64.so.7")         # the already-loaded copy
+        vp, sz = C.c_void_p, C.c_size_t
+        for name, args in {
+            "hipMalloc": [C.POINTER(vp), sz], "hipFree": [vp],
+            "hipMemcpy": [vp, vp, sz, C.c_int], "hipMemset": [vp, C.c_int, sz],
+            "hipDeviceSynchronize": [], "hipGetDeviceCount": [C.POINTER(C.c_int)],
+            "hipStreamCreateWithFlags": [C.POINTER(vp), C.c_uint], "hipStreamDestroy": [vp],
+            "hipStreamSynchronize": [vp], "hipEventCreate": [C.POINTER(vp)],
+            "hipEventDestroy": [vp], "hipEventRecord": [vp, vp], "hipEventSynchronize": [vp],
+            "hipEventElapsedTime": [C.POINTER(C.c_float), vp, vp],
+        }.items():
+            f = getattr(H, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        H.hipGetErrorString.argtypes = [C.c_int]
+        H.hipGetErrorString.restype = C.c_char_p
+        _h = H
+    return _h
+
+
+def check(e, what=""):
+    if e != 0:
+        raise RuntimeError(f"HIP error {e} ({rt().hipGetErrorString(e).decode()}) {what}")
+
+
+def device_count():
+    n = C.c_int(0)
+    if rt().hipGetDeviceCount(C.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+def synchronize():
+    check(rt().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Buffer:
+    """Device buffer of float64."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        check(rt().hipMalloc(C.byref(self.ptr), max(self.n, 1) * 8), "hipMalloc")
+        check(rt().hipMemset(self.ptr, 0, max(self.n, 1) * 8), "hipMemset")
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = cls(a.size)
+        b.upload(a)
+        return b
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        check(rt().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.n * 8, 1), "hipMemcpy H2D")
+
+    def download(self, shape=None):
+        a = np.empty(self.n, dtype=np.float64)
+        check(rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), self.ptr, self.n * 8, 2), "hipMemcpy D2H")
+        return a.reshape(shape) if shape is not None else a
+
+    def free(self):
+        if self.ptr:
+            rt().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A HIP stream at the default priority."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(rt().hipStreamCreateWithFlags(C.byref(self.h), 1), "hipStreamCreate")
+
+    @property
+    def handle(self):
+        return self.h.value
+
+    def synchronize(self):
+        check(rt().hipStreamSynchronize(self.h), "hipStreamSynchronize")
+
+    def destroy(self):
+        if self.h:
+            rt().hipStreamDestroy(self.h)
+            self.h = C.c_void_p()
+
+
+class Event:
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(rt().hipEventCreate(C.byref(self.h)), "hipEventCreate")
+
+    def record(self, stream=None):
+        check(rt().hipEventRecord(self.h, stream.h if stream else None), "hipEventRecord")
+
+    def synchronize(self):
+        check(rt().hipEventSynchronize(self.h), "hipEventSynchronize")
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        check(rt().hipEventElapsedTime(C.byref(ms), self.h, end.h), "hipEventElapsedTime")
+        return float(ms.value)
+
+    def destroy(self):
+        if self.h:
+            rt().hipEventDestroy(self.h)
+            self.h = C.c_void_p()

@@ -1,0 +1,148 @@
+/*
+ * dg_amd.h -- C ABI of the MI355X-native DG residual + RK path.
+ *
+ * Drop-in boundary for the hot path of IhmeGroup/DG-MPI: the 3D Euler
+ * discontinuous-Galerkin residual on periodic hexahedral boxes and the
+ * FE/RK4 stepper that drives it.  The reference has no C ABI; its surface is
+ * the templated C++ Solver<3> over Kokkos views.  Each entry point below
+ * names the reference interface it replaces (paths relative to the
+ * reference repository root).
+ *
+ * Conventions
+ *   - status: every call returns DG_OK (0) or a dg_status code; the message
+ *     is available from dg_last_error(handle) (dg_last_error(NULL) for a
+ *     failed dg_create).  No C++ exception crosses this boundary.
+ *   - ownership: the handle owns its device buffers (solution, RK
+ *     registers); callers own host buffers and the device buffers they pass
+ *     to dg_residual / dg_apply_inv_mass.
+ *   - layout: solution and residual coefficients are FP64, [E][nb][5]
+ *     row-major (element, basis k*(p+1)^2 + j*(p+1) + i, variable rho, rho u,
+ *     rho v, rho w, rho E) -- the reference's LayoutRight Uc
+ *     (src/solver/base.h:47).  DG_LAYOUT_LEFT selects the column-major
+ *     variant the reference uses on GPUs (src/solver/base.cpp:248-275).
+ *   - elements of the periodic box are numbered e = ix + nx*(iy + ny*iz).
+ *   - streams: calls taking a `stream` argument are stream-ordered on that
+ *     HIP stream (NULL = the handle's own stream); the others synchronise
+ *     before returning.  One host thread per handle.
+ */
+#ifndef DG_AMD_H
+#define DG_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dg_solver dg_solver;
+
+enum dg_status {
+    DG_OK = 0,
+    DG_ERR_INPUT = 1,        /* bad case input (reference: InputException)   */
+    DG_ERR_DEVICE = 2,       /* HIP runtime error or no usable GPU           */
+    DG_ERR_ARGUMENT = 3,     /* bad pointer / size / layout argument         */
+    DG_ERR_UNSUPPORTED = 4,  /* valid reference input outside this path      */
+    DG_ERR_NUMERICS = 5      /* non-finite state (the reference asserts)     */
+};
+
+enum dg_layout { DG_LAYOUT_RIGHT = 0, DG_LAYOUT_LEFT = 1 };
+
+typedef struct {
+    long num_elems;          /* E                                            */
+    int order;               /* p                                            */
+    int num_basis;           /* (p+1)^3                                      */
+    int num_state_vars;      /* 5                                            */
+    int nelem[3];
+    double lower[3], upper[3];
+    double gamma, gas_constant;
+    double timestep;         /* Stepper.timestep, or the CFL-derived dt      */
+    double end_time;         /* Stepper.end                                  */
+    double time;             /* solver time                                  */
+    int num_iterations;      /* (int)(end / timestep), as Solver::solve      */
+    int stepper;             /* 0 FE, 1 RK4                                  */
+    int flux;                /* 0 LaxFriedrichs, 1 HLLC, 2 Roe               */
+    int basis;               /* 0 LagrangeHex, 1 LegendreHex                 */
+    int ic;                  /* 0 Uniform, 1 Sphere, 2 IsentropicVortex,
+                                3 ExtrudedIsentropicVortex, 4 GaussianBump,
+                                5 TaylorGreen                                */
+} dg_info;
+
+/* Parse a TOML case (the keys of src/exec/main.cpp:38-83,
+ * src/solver/base.cpp:20-67, src/physics/base/base.cpp:24-32,
+ * src/numerics/numerics_data.cpp:229-300; mesh keys: see INTEGRATION.md),
+ * build the periodic mesh and the reference-element operators
+ * (Solver<dim>::Solver + precompute_matrix_helpers,
+ * src/solver/base.cpp:20-111) and allocate device state on the current
+ * HIP device.  The state is left at zero; call dg_init_state. */
+int dg_create(const char* toml_text, dg_solver** out);
+
+/* Release device buffers and the handle.  Accepts NULL. */
+int dg_destroy(dg_solver* s);
+
+/* Last error message of the handle (NULL: of the last failed dg_create on
+ * this thread).  Never returns NULL. */
+const char* dg_last_error(const dg_solver* s);
+
+/* Sizes and parameters of the case. */
+int dg_get_info(const dg_solver* s, dg_info* info);
+
+/* L2 projection of the initial condition on (2p+2)^3 Gauss points, time
+ * set to 0 (Solver::init_state_from_fcn, src/solver/base.cpp:113-223;
+ * SolverTools::L2_projection, src/solver/tools.cpp:66-122). */
+int dg_init_state(dg_solver* s);
+
+/* Host <-> device copies of the solution coefficients (the h_Uc mirror,
+ * src/solver/base.cpp:226-230, and the two layouts of
+ * read_in_coefficients, src/solver/base.cpp:233-279).  host holds
+ * E*(p+1)^3*5 doubles. */
+int dg_set_state(dg_solver* s, const double* host, int layout);
+int dg_get_state(const dg_solver* s, double* host, int layout);
+
+/* Device pointer to the solution coefficients ([E][nb][5], owned by the
+ * handle; valid until dg_destroy). */
+int dg_state_device_ptr(dg_solver* s, double** dev);
+
+/* Residual R(U) = volume term - face term, not mass-inverted
+ * (Solver::get_residual, src/solver/base.cpp:401-561).  U_dev and R_dev
+ * are caller-owned device buffers of E*nb*5 doubles and must not alias. */
+int dg_residual(dg_solver* s, const double* U_dev, double* R_dev, void* stream);
+
+/* dU = dt * M^-1 R per element (SolverTools::mult_inv_mass_matrix,
+ * src/solver/tools.cpp:124-139).  In-place (R_dev == dU_dev) is allowed. */
+int dg_apply_inv_mass(dg_solver* s, double dt, const double* R_dev, double* dU_dev,
+                      void* stream);
+
+/* nsteps time steps of the configured stepper on the handle's state
+ * (StepperBase::take_time_step: FE src/numerics/timestepping/stepper.cpp:23-55,
+ * RK4 :71-142).  dt <= 0 uses the case time step. */
+int dg_step(dg_solver* s, double dt, int nsteps);
+
+/* Solver::solve (src/solver/base.cpp:377-399): (int)(end/dt) steps. */
+int dg_solve(dg_solver* s);
+
+/* Per-variable L2 error against the exact solution at time t, normalised
+ * by the domain volume, on (2p+2)^3 points (get_error with ord = 2,
+ * src/post/main.cpp:30-201). */
+int dg_l2_error(dg_solver* s, double t, double err[5]);
+
+/* Domain integrals of rho, rho u, rho v, rho w, rho E and of the kinetic
+ * energy 1/2 rho |u|^2 on (2p+2)^3 points, compensated FP64 sums (no
+ * reference counterpart; conservation and kinetic-energy diagnostics). */
+int dg_integrals(dg_solver* s, double out[6]);
+
+/* Largest |u| + c over the volume quadrature points of the state (the
+ * wave speed of src/physics/euler/functions.cpp:25-40); the CFL rule
+ * dt = CFL * min(h) / max(|u|+c) (Stepper.cfl) uses it. */
+int dg_max_wavespeed(dg_solver* s, double* amax);
+
+/* Set the solver time (the reference hard-codes 0, src/solver/base.cpp:25). */
+int dg_set_time(dg_solver* s, double t);
+
+/* Make the handle launch on a caller-owned HIP stream (NULL: back to the
+ * handle's own stream).  No reference counterpart (Kokkos uses the default
+ * execution space instance); lets callers order and time the path's
+ * kernels on their own stream. */
+int dg_set_stream(dg_solver* s, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DG_AMD_H */
