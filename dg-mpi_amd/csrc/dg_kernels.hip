@@ -35,6 +35,11 @@ namespace {
 
 constexpr int V = NVAR;
 constexpr int WAVE = 64;
+// threads that cooperate on one element in the fused stage kernel
+#ifndef DG_STAGE_TEAM
+#define DG_STAGE_TEAM 128
+#endif
+constexpr int TEAM = DG_STAGE_TEAM;
 
 template <int P>
 struct Dims {
@@ -43,10 +48,10 @@ struct Dims {
     static constexpr int Q1 = 2 * P + 2;
 };
 
-template <int T, class F>
+template <int T, int TS = WAVE, class F>
 __device__ __forceinline__ void team_for(int lane, F&& f) {
 #pragma unroll
-    for (int t0 = 0; t0 < T; t0 += WAVE) {
+    for (int t0 = 0; t0 < T; t0 += TS) {
         const int t = t0 + lane;
         if (t < T) f(t);
     }
@@ -66,18 +71,22 @@ __device__ __forceinline__ double pressure(double gam, const double* U) {
     return (gam - 1.0) * (U[4] - 0.5 * rKE);
 }
 
+// Pressure with 1/rho supplied (one reciprocal per state, reused below).
+__device__ __forceinline__ double pressure_r(double gam, const double* U, double r1) {
+    return (gam - 1.0) * (U[4] - 0.5 * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * r1));
+}
+
 // src/physics/euler/functions.cpp:25-40
 __device__ __forceinline__ double max_wave_speed(double gam, const double* U) {
     const double rho1 = 1. / U[0];
     const double sqrtrKE = sqrt(U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * rho1;
-    return sqrtrKE + sqrt(gam * pressure(gam, U) * rho1);
+    return sqrtrKE + sqrt(gam * pressure_r(gam, U, rho1) * rho1);
 }
 
-// Euler flux along axis D (one block of src/physics/euler/functions.cpp:64-91).
+// Euler flux along axis D (one block of src/physics/euler/functions.cpp:64-91)
+// with 1/rho and p supplied.
 template <int D>
-__device__ __forceinline__ void flux_axis(double gam, const double* U, double* F) {
-    const double r1 = 1. / U[0];
-    const double P = pressure(gam, U);
+__device__ __forceinline__ void flux_axis(const double* U, double r1, double P, double* F) {
     const double un = U[1 + D];
     F[0] = un;
 #pragma unroll
@@ -92,11 +101,13 @@ __device__ __forceinline__ void flux_axis(double gam, const double* U, double* F
 template <int D>
 __device__ __forceinline__ void riemann_llf(double gam, const double* UL, const double* UR,
                                             double nmag, double* F) {
+    const double r1L = 1. / UL[0], r1R = 1. / UR[0];
+    const double PL = pressure_r(gam, UL, r1L), PR = pressure_r(gam, UR, r1R);
     double FL[V], FR[V];
-    flux_axis<D>(gam, UL, FL);
-    flux_axis<D>(gam, UR, FR);
-    double a = max_wave_speed(gam, UL);
-    const double aR = max_wave_speed(gam, UR);
+    flux_axis<D>(UL, r1L, PL, FL);
+    flux_axis<D>(UR, r1R, PR, FR);
+    double a = sqrt(UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) * r1L + sqrt(gam * PL * r1L);
+    const double aR = sqrt(UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]) * r1R + sqrt(gam * PR * r1R);
     if (aR > a) a = aR;
 #pragma unroll
     for (int s = 0; s < V; ++s) F[s] = nmag * (0.5 * (FL[s] + FR[s]) - 0.5 * a * (UR[s] - UL[s]));
@@ -109,19 +120,22 @@ __device__ __forceinline__ void riemann_hllc(double gam, const double* UL, const
     const double djac = nmag;
     const double rhoL = UL[0], rhoL1 = 1. / rhoL;
     const double unL = UL[1 + D] * rhoL1;
-    const double PL = pressure(gam, UL);
+    const double PL = pressure_r(gam, UL, rhoL1);
     const double aL = sqrt(gam * PL * rhoL1);
     const double rhoR = UR[0], rhoR1 = 1. / rhoR;
     const double unR = UR[1 + D] * rhoR1;
-    const double PR = pressure(gam, UR);
+    const double PR = pressure_r(gam, UR, rhoR1);
     const double aR = sqrt(gam * PR * rhoR1);
     const double rAvg = 0.5 * (rhoL + rhoR);
     const double aAvg = 0.5 * (aL + aR);
     const double pStar = fmax(0., 0.5 * (PL + PR - (unR - unL) * rAvg * aAvg));
+    const double gq = (gam + 1.) / (2. * gam);
     double qL = 1.0;
-    if (pStar / PL > 1.) qL = sqrt(1. + (gam + 1.) / (2. * gam) * (pStar / PL - 1.));
+    const double xL = pStar / PL;
+    if (xL > 1.) qL = sqrt(1. + gq * (xL - 1.));
     double qR = 1.0;
-    if (pStar / PR > 1.) qR = sqrt(1. + (gam + 1.) / (2. * gam) * (pStar / PR - 1.));
+    const double xR = pStar / PR;
+    if (xR > 1.) qR = sqrt(1. + gq * (xR - 1.));
     const double sL = unL - aL * qL;
     const double sR = unR + aR * qR;
     const double sStar = (PR - PL + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR)) /
@@ -175,62 +189,64 @@ __device__ __forceinline__ void riemann_hllc(double gam, const double* UL, const
 }
 
 // Roe-Pike flux with a Harten-Hyman-type entropy fix on the acoustic waves
-// (no reference counterpart; same algorithm as the oracle's flux_roe).
+// (no reference counterpart; the algorithm of the oracle's flux_roe,
+// arranged around four reciprocals and five square roots).
 __device__ __forceinline__ double entropy_fix(double lam, double lamL, double lamR) {
     const double delta = fmax(0., fmax(lam - lamL, lamR - lam));
-    const double a = fabs(lam);
-    return a < delta ? 0.5 * (lam * lam + delta * delta) / delta : a;
+    double a = fabs(lam);
+    if (a < delta) a = 0.5 * (lam * lam + delta * delta) / delta;   // rare: transonic expansion
+    return a;
 }
 
 template <int D>
 __device__ __forceinline__ void riemann_roe(double gam, const double* UL, const double* UR,
                                             double nmag, double* F) {
     const double rL = UL[0], rR = UR[0];
-    const double uL[3] = {UL[1] / rL, UL[2] / rL, UL[3] / rL};
-    const double uR[3] = {UR[1] / rR, UR[2] / rR, UR[3] / rR};
-    const double pL = pressure(gam, UL), pR = pressure(gam, UR);
-    const double HL = (UL[4] + pL) / rL, HR = (UR[4] + pR) / rR;
+    const double r1L = 1. / rL, r1R = 1. / rR;
+    double uL[3], uR[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        uL[d] = UL[1 + d] * r1L;
+        uR[d] = UR[1 + d] * r1R;
+    }
+    const double gm1 = gam - 1.;
+    const double pL = gm1 * (UL[4] - 0.5 * (UL[1] * uL[0] + UL[2] * uL[1] + UL[3] * uL[2]));
+    const double pR = gm1 * (UR[4] - 0.5 * (UR[1] * uR[0] + UR[2] * uR[1] + UR[3] * uR[2]));
+    const double HL = (UL[4] + pL) * r1L, HR = (UR[4] + pR) * r1R;
     const double sL = sqrt(rL), sR = sqrt(rR), isum = 1. / (sL + sR);
     double ut[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) ut[d] = (sL * uL[d] + sR * uR[d]) * isum;
     const double Ht = (sL * HL + sR * HR) * isum;
     const double q2 = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
-    const double c2 = (gam - 1.) * (Ht - 0.5 * q2);
+    const double c2 = gm1 * (Ht - 0.5 * q2);
     const double ct = sqrt(c2);
+    const double ic2 = 1. / c2;
     const double rt = sL * sR;
     const double unt = ut[D], unL = uL[D], unR = uR[D];
     const double drho = rR - rL, dp = pR - pL, dun = unR - unL;
-    const double a1 = (dp - rt * ct * dun) / (2. * c2);
-    const double a5 = (dp + rt * ct * dun) / (2. * c2);
-    const double a2 = drho - dp / c2;
-    const double cL = sqrt(gam * pL / rL), cR = sqrt(gam * pR / rR);
+    const double a1 = (dp - rt * ct * dun) * (0.5 * ic2);
+    const double a5 = (dp + rt * ct * dun) * (0.5 * ic2);
+    const double a2 = drho - dp * ic2;
+    const double cL = sqrt(gam * pL * r1L), cR = sqrt(gam * pR * r1R);
     const double l1 = entropy_fix(unt - ct, unL - cL, unR - cR);
     const double l2 = fabs(unt);
     const double l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
+    const double w1 = l1 * a1, w5 = l5 * a5, w2 = l2 * a2, ws = l2 * rt;
     double Dv[V];
-    Dv[0] = l1 * a1 + l5 * a5;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double cn = (d == D) ? ct : 0.0;
-        Dv[1 + d] = l1 * a1 * (ut[d] - cn) + l5 * a5 * (ut[d] + cn);
-    }
-    Dv[4] = l1 * a1 * (Ht - unt * ct) + l5 * a5 * (Ht + unt * ct);
-    Dv[0] += l2 * a2;
-#pragma unroll
-    for (int d = 0; d < 3; d++) Dv[1 + d] += l2 * a2 * ut[d];
-    Dv[4] += l2 * a2 * 0.5 * q2;
+    Dv[0] = w1 + w5 + w2;
     double udu = 0.;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
+        const double cn = (d == D) ? ct : 0.0;
         const double dut = (d == D) ? (uR[d] - uL[d]) - dun : (uR[d] - uL[d]);
-        Dv[1 + d] += l2 * rt * dut;
+        Dv[1 + d] = w1 * (ut[d] - cn) + w5 * (ut[d] + cn) + w2 * ut[d] + ws * dut;
         udu += ut[d] * dut;
     }
-    Dv[4] += l2 * rt * udu;
+    Dv[4] = w1 * (Ht - unt * ct) + w5 * (Ht + unt * ct) + w2 * 0.5 * q2 + ws * udu;
     double FL[V], FR[V];
-    flux_axis<D>(gam, UL, FL);
-    flux_axis<D>(gam, UR, FR);
+    flux_axis<D>(UL, r1L, pL, FL);
+    flux_axis<D>(UR, r1R, pR, FR);
 #pragma unroll
     for (int s = 0; s < V; ++s) F[s] = nmag * (0.5 * (FL[s] + FR[s]) - 0.5 * Dv[s]);
 }
@@ -374,14 +390,16 @@ template <int P>
 struct StageLDS {
     using Dm = Dims<P>;
     static constexpr int N = Dm::N, M = Dm::M;
-    // face phase (per direction)
-    static constexpr int U = Dm::NNN * V;       // own coefficients
-    static constexpr int SLAB = 4 * Dm::NN * V;  // own-, own+, nbr-, nbr+
-    static constexpr int F1 = 4 * N * M * V;
-    static constexpr int F2 = 4 * Dm::MM * V;
-    static constexpr int FH = 2 * Dm::MM * V;
-    static constexpr int L1 = 2 * N * M * V;
-    static constexpr int L2 = 2 * Dm::NN * V;
+    // face phase, all three directions at once; "sl" indexes the four
+    // traces of a direction: own on the - face, own on the + face,
+    // neighbour across the - face, neighbour across the + face
+    static constexpr int U = Dm::NNN * V;            // own coefficients
+    static constexpr int SLAB = 12 * Dm::NN * V;      // [d][sl][t2][t1][s]
+    static constexpr int F1 = 12 * N * M * V;         // [d][sl][t2][a1][s]
+    static constexpr int F2 = 12 * Dm::MM * V;        // [d][sl][a2][a1][s]
+    static constexpr int FH = 6 * Dm::MM * V;         // [d][side][a2][a1][s]
+    static constexpr int L1 = 6 * N * M * V;          // [d][side][t2][a1][s]
+    static constexpr int L2 = 6 * Dm::NN * V;         // [d][side][t2][t1][s]
     // volume phase
     static constexpr int X1 = N * N * M * V;
     static constexpr int X2 = N * M * M * V;
@@ -394,13 +412,28 @@ struct StageLDS {
     static constexpr int Bk = cmax(cmax(cmax(SLAB, F2), L1), cmax(cmax(X1, UQ), C));
     static constexpr int R = Dm::NNN * V;
     static constexpr int TOTAL = R + A + Bk;
+    // Riemann tasks per direction, padded to whole waves so that every
+    // wave works on a single direction
+    static constexpr int RP = ((2 * Dm::MM + WAVE - 1) / WAVE) * WAVE;
 };
 
+// basis index of (normal index r, tangential t1, t2) for faces normal to d:
+// d = 0: (i, j, k) = (r, t1, t2); d = 1: (t1, r, t2); d = 2: (t1, t2, r)
+template <int N>
+__device__ __forceinline__ int bidx_d(int d, int r, int t1, int t2) {
+    const int sr = d == 0 ? 1 : (d == 1 ? N : N * N);
+    const int s1 = d == 0 ? N : 1;
+    const int s2 = d == 2 ? N : N * N;
+    return r * sr + t1 * s1 + t2 * s2;
+}
+
 // ---------------------------------------------------------------------------
-// Face phase for faces normal to D: traces (own and neighbour), Riemann
-// flux, lift, accumulated into sR.
+// Face phase: traces (own and neighbour) on all six faces, Riemann fluxes,
+// lift, accumulated into sR.  The neighbour slab uses exactly the
+// arithmetic of the neighbour's own slab, so both sides of a face see the
+// same trace and the same flux.
 // ---------------------------------------------------------------------------
-template <int P, int FLUX, int D>
+template <int P, int FLUX, int TS>
 __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T, const KGeom& G,
                                            const int* ec, int lane, const double* sU, double* sR,
                                            double* bankA, double* bankB) {
@@ -414,82 +447,85 @@ __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T,
     double* l1 = bankB;
     double* l2 = bankA + L::U;
 
-    const unsigned nm = elem_at(ec[0] - (D == 0), ec[1] - (D == 1), ec[2] - (D == 2), G);
-    const unsigned np = elem_at(ec[0] + (D == 0), ec[1] + (D == 1), ec[2] + (D == 2), G);
-
-    // 1. normal contractions: own (both sides) and neighbour slabs.  The
-    //    neighbour slab uses the arithmetic of the neighbour's own slab.
-    team_for<NN * V>(lane, [&](int t) {
-        const int s = t % V, tt = t / V, t1 = tt % N, t2 = tt / N;
+    // 1. normal contractions: neighbour slabs (global loads issued first),
+    //    then own slabs of both sides
+    team_for<6 * NN * V, TS>(lane, [&](int t) {
+        const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
+        const int d = ds >> 1, side = ds & 1;
+        const int off = side ? 1 : -1;
+        const unsigned e = elem_at(ec[0] + (d == 0 ? off : 0), ec[1] + (d == 1 ? off : 0),
+                                   ec[2] + (d == 2 ? off : 0), G);
+        const double* Un = A.Ust + (size_t)e * (NNN * V);
+        // neighbour across the - face contributes its + trace and vice versa
+        const double* phi = side == 0 ? T.phip : T.phim;
+        double acc = 0.;
+#pragma unroll
+        for (int r = 0; r < N; ++r) acc = fma(phi[r], Un[bidx_d<N>(d, r, t1, t2) * V + s], acc);
+        slab[((d * 4 + 2 + side) * NN + t2 * N + t1) * V + s] = acc;
+    });
+    team_for<3 * NN * V, TS>(lane, [&](int t) {
+        const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, d = u2 / N;
         double um = 0., up = 0.;
 #pragma unroll
         for (int r = 0; r < N; ++r) {
-            const double u = sU[bidx<N, D>(r, t1, t2) * V + s];
-            um = fma(T.phim[r], u, um);
-            up = fma(T.phip[r], u, up);
+            const double x = sU[bidx_d<N>(d, r, t1, t2) * V + s];
+            um = fma(T.phim[r], x, um);
+            up = fma(T.phip[r], x, up);
         }
-        slab[(0 * NN + t2 * N + t1) * V + s] = um;
-        slab[(1 * NN + t2 * N + t1) * V + s] = up;
-    });
-    team_for<2 * NN * V>(lane, [&](int t) {
-        const int s = t % V, tt = t / V, t1 = tt % N, t3 = tt / N, t2 = t3 % N, side = t3 / N;
-        const double* Un = A.Ust + (size_t)(side == 0 ? nm : np) * (NNN * V);
-        double acc = 0.;
-        if (side == 0) {
-#pragma unroll
-            for (int r = 0; r < N; ++r) acc = fma(T.phip[r], Un[bidx<N, D>(r, t1, t2) * V + s], acc);
-        } else {
-#pragma unroll
-            for (int r = 0; r < N; ++r) acc = fma(T.phim[r], Un[bidx<N, D>(r, t1, t2) * V + s], acc);
-        }
-        slab[((2 + side) * NN + t2 * N + t1) * V + s] = acc;
+        slab[((d * 4 + 0) * NN + t2 * N + t1) * V + s] = um;
+        slab[((d * 4 + 1) * NN + t2 * N + t1) * V + s] = up;
     });
     sync();
     // 2. interpolate along t1 to the face quadrature points
-    team_for<4 * N * V>(lane, [&](int t) {
-        const int s = t % V, tt = t / V, t2 = tt % N, sl = tt / N;
-        double u[N];
+    team_for<12 * N * V, TS>(lane, [&](int t) {
+        const int s = t % V, u = t / V, t2 = u % N, ds = u / N;
+        double x[N];
 #pragma unroll
-        for (int t1 = 0; t1 < N; ++t1) u[t1] = slab[(sl * NN + t2 * N + t1) * V + s];
+        for (int t1 = 0; t1 < N; ++t1) x[t1] = slab[(ds * NN + t2 * N + t1) * V + s];
 #pragma unroll
         for (int a = 0; a < M; ++a) {
             double acc = 0.;
 #pragma unroll
-            for (int t1 = 0; t1 < N; ++t1) acc = fma(T.B[a][t1], u[t1], acc);
-            f1[((sl * N + t2) * M + a) * V + s] = acc;
+            for (int t1 = 0; t1 < N; ++t1) acc = fma(T.B[a][t1], x[t1], acc);
+            f1[((ds * N + t2) * M + a) * V + s] = acc;
         }
     });
     sync();
     // 3. interpolate along t2
-    team_for<4 * M * V>(lane, [&](int t) {
-        const int s = t % V, tt = t / V, a1 = tt % M, sl = tt / M;
-        double u[N];
+    team_for<12 * M * V, TS>(lane, [&](int t) {
+        const int s = t % V, u = t / V, a1 = u % M, ds = u / M;
+        double x[N];
 #pragma unroll
-        for (int t2 = 0; t2 < N; ++t2) u[t2] = f1[((sl * N + t2) * M + a1) * V + s];
+        for (int t2 = 0; t2 < N; ++t2) x[t2] = f1[((ds * N + t2) * M + a1) * V + s];
 #pragma unroll
         for (int a2 = 0; a2 < M; ++a2) {
             double acc = 0.;
 #pragma unroll
-            for (int t2 = 0; t2 < N; ++t2) acc = fma(T.B[a2][t2], u[t2], acc);
-            f2[((sl * M + a2) * M + a1) * V + s] = acc;
+            for (int t2 = 0; t2 < N; ++t2) acc = fma(T.B[a2][t2], x[t2], acc);
+            f2[((ds * M + a2) * M + a1) * V + s] = acc;
         }
     });
     sync();
-    // 4. Riemann flux at the face points; sign and face weight folded in
-    //    (Fq = F w, Fq_elem = -Fq on the left element, +Fq on the right)
-    team_for<2 * MM>(lane, [&](int t) {
-        const int a1 = t % M, tt = t / M, a2 = tt % M, side = tt / M;
-        const double* own = f2 + ((side * M + a2) * M + a1) * V;
-        const double* nb = f2 + (((2 + side) * M + a2) * M + a1) * V;
+    // 4. Riemann flux at the face points; face weight and the sign of the
+    //    element's side folded in (Fq = F w; -Fq on the left element, +Fq
+    //    on the right).  Each wave handles one direction.
+    team_for<3 * L::RP, TS>(lane, [&](int t) {
+        const int d = t / L::RP, q = t % L::RP;
+        if (q >= 2 * MM) return;
+        const int a1 = q % M, u = q / M, a2 = u % M, side = u / M;
+        const double* own = f2 + (((d * 4 + side) * M + a2) * M + a1) * V;
+        const double* nbv = f2 + (((d * 4 + 2 + side) * M + a2) * M + a1) * V;
         double UL[V], UR[V], F[V];
 #pragma unroll
         for (int s = 0; s < V; ++s) {
-            UL[s] = side == 0 ? nb[s] : own[s];
-            UR[s] = side == 0 ? own[s] : nb[s];
+            UL[s] = side == 0 ? nbv[s] : own[s];
+            UR[s] = side == 0 ? own[s] : nbv[s];
         }
-        riemann<FLUX, D>(G.gamma, UL, UR, G.farea[D], F);
+        if (d == 0) riemann<FLUX, 0>(G.gamma, UL, UR, G.farea[0], F);
+        else if (d == 1) riemann<FLUX, 1>(G.gamma, UL, UR, G.farea[1], F);
+        else riemann<FLUX, 2>(G.gamma, UL, UR, G.farea[2], F);
         const double wq = T.w[a1] * T.w[a2];
-        double* o = fh + ((side * M + a2) * M + a1) * V;
+        double* o = fh + (((d * 2 + side) * M + a2) * M + a1) * V;
 #pragma unroll
         for (int s = 0; s < V; ++s) {
             const double fw = F[s] * wq;
@@ -498,44 +534,45 @@ __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T,
     });
     sync();
     // 5. lift: transpose of the interpolation, along t2 then t1
-    team_for<2 * M * V>(lane, [&](int t) {
-        const int s = t % V, tt = t / V, a1 = tt % M, side = tt / M;
+    team_for<6 * M * V, TS>(lane, [&](int t) {
+        const int s = t % V, u = t / V, a1 = u % M, ds = u / M;
         double f[M];
 #pragma unroll
-        for (int a2 = 0; a2 < M; ++a2) f[a2] = fh[((side * M + a2) * M + a1) * V + s];
+        for (int a2 = 0; a2 < M; ++a2) f[a2] = fh[((ds * M + a2) * M + a1) * V + s];
 #pragma unroll
         for (int t2 = 0; t2 < N; ++t2) {
             double acc = 0.;
 #pragma unroll
             for (int a2 = 0; a2 < M; ++a2) acc = fma(T.B[a2][t2], f[a2], acc);
-            l1[((side * N + t2) * M + a1) * V + s] = acc;
+            l1[((ds * N + t2) * M + a1) * V + s] = acc;
         }
     });
     sync();
-    team_for<2 * N * V>(lane, [&](int t) {
-        const int s = t % V, tt = t / V, t2 = tt % N, side = tt / N;
+    team_for<6 * N * V, TS>(lane, [&](int t) {
+        const int s = t % V, u = t / V, t2 = u % N, ds = u / N;
         double f[M];
 #pragma unroll
-        for (int a1 = 0; a1 < M; ++a1) f[a1] = l1[((side * N + t2) * M + a1) * V + s];
+        for (int a1 = 0; a1 < M; ++a1) f[a1] = l1[((ds * N + t2) * M + a1) * V + s];
 #pragma unroll
         for (int t1 = 0; t1 < N; ++t1) {
             double acc = 0.;
 #pragma unroll
             for (int a1 = 0; a1 < M; ++a1) acc = fma(T.B[a1][t1], f[a1], acc);
-            l2[((side * N + t2) * N + t1) * V + s] = acc;
+            l2[((ds * N + t2) * N + t1) * V + s] = acc;
         }
     });
     sync();
-    // 6. expand along the normal direction into the residual
-    team_for<NN * V>(lane, [&](int t) {
-        const int s = t % V, tt = t / V, t1 = tt % N, t2 = tt / N;
-        const double lm = l2[((0 * N + t2) * N + t1) * V + s];
-        const double lp = l2[((1 * N + t2) * N + t1) * V + s];
-#pragma unroll
-        for (int r = 0; r < N; ++r) {
-            double* o = sR + bidx<N, D>(r, t1, t2) * V + s;
-            *o += T.phim[r] * lm + T.phip[r] * lp;
-        }
+    // 6. expand along the normal directions into the residual
+    team_for<NNN * V, TS>(lane, [&](int t) {
+        const int s = t % V, b = t / V, i = b % N, j = (b / N) % N, k = b / NN;
+        double acc = sR[t];
+        // d = 0: r = i, (t1, t2) = (j, k)
+        acc += T.phim[i] * l2[((0 * N + k) * N + j) * V + s] + T.phip[i] * l2[((1 * N + k) * N + j) * V + s];
+        // d = 1: r = j, (t1, t2) = (i, k)
+        acc += T.phim[j] * l2[((2 * N + k) * N + i) * V + s] + T.phip[j] * l2[((3 * N + k) * N + i) * V + s];
+        // d = 2: r = k, (t1, t2) = (i, j)
+        acc += T.phim[k] * l2[((4 * N + j) * N + i) * V + s] + T.phip[k] * l2[((5 * N + j) * N + i) * V + s];
+        sR[t] = acc;
     });
     sync();
 }
@@ -543,7 +580,7 @@ __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T,
 // ---------------------------------------------------------------------------
 // Volume phase: sum-factorised interpolation, point fluxes, weak gradient.
 // ---------------------------------------------------------------------------
-template <int P>
+template <int P, int TS>
 __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, int lane,
                                              double* sR, double* bankA, double* bankB) {
     using Dm = Dims<P>;
@@ -557,7 +594,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     double* Bv = bankA;
 
     // x: X1[k][j][a][s] = sum_i B[a][i] U[k][j][i][s]
-    team_for<NN * V>(lane, [&](int t) {
+    team_for<NN * V, TS>(lane, [&](int t) {
         const int s = t % V, kj = t / V;
         double u[N];
 #pragma unroll
@@ -572,7 +609,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     });
     sync();
     // y: X2[k][b][a][s] = sum_j B[b][j] X1[k][j][a][s]
-    team_for<N * M * V>(lane, [&](int t) {
+    team_for<N * M * V, TS>(lane, [&](int t) {
         const int s = t % V, tt = t / V, a = tt % M, k = tt / M;
         double u[N];
 #pragma unroll
@@ -587,7 +624,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     });
     sync();
     // z: Uq[c][b][a][s] = sum_k B[c][k] X2[k][b][a][s]
-    team_for<MM * V>(lane, [&](int t) {
+    team_for<MM * V, TS>(lane, [&](int t) {
         const int s = t % V, ba = t / V;
         double u[N];
 #pragma unroll
@@ -602,13 +639,13 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     });
     sync();
     // point data: (rho, m_x, m_y, m_z, E + p, u, v, w, p)
-    team_for<MMM>(lane, [&](int q) {
+    team_for<MMM, TS>(lane, [&](int q) {
         const double* u = Uq + q * V;
         double U[V];
 #pragma unroll
         for (int s = 0; s < V; ++s) U[s] = u[s];
-        const double p = pressure(G.gamma, U);
         const double r1 = 1. / U[0];
+        const double p = pressure_r(G.gamma, U, r1);
         double* o = Qd + q * 9;
         o[0] = U[0];
         o[1] = U[1];
@@ -623,7 +660,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     sync();
     // c-step: contravariant fluxes G_d = F_d (dxi_d/dx_d) |J| w, contracted
     // along z: B for x and y fluxes, D for the z flux
-    team_for<MM * V>(lane, [&](int t) {
+    team_for<MM * V, TS>(lane, [&](int t) {
         const int s = t % V, ba = t / V, a = ba % M, b = ba / M;
         const double wab = T.w[a] * T.w[b];
         double xx[N], yy[N], zz[N];
@@ -653,7 +690,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     });
     sync();
     // b-step: along y (B for x and z fluxes, D for the y flux)
-    team_for<N * M * V>(lane, [&](int t) {
+    team_for<N * M * V, TS>(lane, [&](int t) {
         const int s = t % V, tt = t / V, a = tt % M, k = tt / M;
         double cx[M], cy[M], cz[M];
 #pragma unroll
@@ -677,7 +714,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     });
     sync();
     // a-step: along x (D for the x flux, B for the others), into sR
-    team_for<NN * V>(lane, [&](int t) {
+    team_for<NN * V, TS>(lane, [&](int t) {
         const int s = t % V, kj = t / V;
         double y1[M], y2[M];
 #pragma unroll
@@ -702,12 +739,12 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
 // Apply M^-1 (tensor product of 1D inverses, scaled by 1/|J|) in place on
 // an LDS coefficient block (without the dt and 1/|J| factors for the
 // non-diagonal case; the caller scales).
-template <int P>
+template <int P, int TS>
 __device__ __forceinline__ void minv_lines(const KTables& T, int lane, double* sR) {
     using Dm = Dims<P>;
     constexpr int N = Dm::N, NN = Dm::NN;
     // x lines
-    team_for<NN * V>(lane, [&](int t) {
+    team_for<NN * V, TS>(lane, [&](int t) {
         const int s = t % V, kj = t / V;
         double u[N];
 #pragma unroll
@@ -722,7 +759,7 @@ __device__ __forceinline__ void minv_lines(const KTables& T, int lane, double* s
     });
     sync();
     // y lines
-    team_for<NN * V>(lane, [&](int t) {
+    team_for<NN * V, TS>(lane, [&](int t) {
         const int s = t % V, ki = t / V, i = ki % N, k = ki / N;
         double u[N];
 #pragma unroll
@@ -737,7 +774,7 @@ __device__ __forceinline__ void minv_lines(const KTables& T, int lane, double* s
     });
     sync();
     // z lines
-    team_for<NN * V>(lane, [&](int t) {
+    team_for<NN * V, TS>(lane, [&](int t) {
         const int s = t % V, ji = t / V;
         double u[N];
 #pragma unroll
@@ -765,7 +802,9 @@ __device__ __forceinline__ double minv_diag_scale(const KTables& T, int b) {
 // Fused stage kernel
 // ---------------------------------------------------------------------------
 template <int P, int FLUX>
-__global__ __launch_bounds__(WAVE) void stage_kernel(StageArgs A, KTables T, KGeom G) {
+__global__ __launch_bounds__(TEAM) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp, KGeom G) {
+    constexpr int TS = TEAM;
+    const KTables& T = *Tp;
     using Dm = Dims<P>;
     using L = StageLDS<P>;
     constexpr int NNN = Dm::NNN;
@@ -780,25 +819,23 @@ __global__ __launch_bounds__(WAVE) void stage_kernel(StageArgs A, KTables T, KGe
     elem_ijk(e, G, ec);
 
     const double* Ue = A.Ust + (size_t)e * NB;
-    team_for<NB>(lane, [&](int t) {
+    team_for<NB, TS>(lane, [&](int t) {
         bankA[t] = Ue[t];
         sR[t] = 0.;
     });
     sync();
-    face_phase<P, FLUX, 0>(A, T, G, ec, lane, bankA, sR, bankA, bankB);
-    face_phase<P, FLUX, 1>(A, T, G, ec, lane, bankA, sR, bankA, bankB);
-    face_phase<P, FLUX, 2>(A, T, G, ec, lane, bankA, sR, bankA, bankB);
-    volume_phase<P>(T, G, lane, sR, bankA, bankB);
+    face_phase<P, FLUX, TS>(A, T, G, ec, lane, bankA, sR, bankA, bankB);
+    volume_phase<P, TS>(T, G, lane, sR, bankA, bankB);
 
     const size_t base = (size_t)e * NB;
     if (A.mode == MODE_RESIDUAL) {
-        team_for<NB>(lane, [&](int t) { A.R[base + t] = sR[t]; });
+        team_for<NB, TS>(lane, [&](int t) { A.R[base + t] = sR[t]; });
         return;
     }
     double scale = A.dt * G.jinv;
-    if (!T.minv_diag) minv_lines<P>(T, lane, sR);
+    if (!T.minv_diag) minv_lines<P, TS>(T, lane, sR);
     const int mode = A.mode;
-    team_for<NB>(lane, [&](int t) {
+    team_for<NB, TS>(lane, [&](int t) {
         const double sc = T.minv_diag ? scale * minv_diag_scale<P>(T, t / V) : scale;
         const double dU = sc * sR[t];
         const size_t g = base + t;
@@ -819,14 +856,15 @@ __global__ __launch_bounds__(WAVE) void stage_kernel(StageArgs A, KTables T, KGe
 // dU = dt M^-1 R
 template <int P>
 __global__ __launch_bounds__(WAVE) void inv_mass_kernel(double dt, const double* R, double* dU,
-                                                        KTables T, KGeom G) {
+                                                        const KTables* __restrict__ Tp, KGeom G) {
+    const KTables& T = *Tp;
     constexpr int NB = Dims<P>::NNN * V;
     __shared__ double sR[NB];
     const int lane = threadIdx.x;
     const size_t base = (size_t)blockIdx.x * NB;
     team_for<NB>(lane, [&](int t) { sR[t] = R[base + t]; });
     sync();
-    if (!T.minv_diag) minv_lines<P>(T, lane, sR);
+    if (!T.minv_diag) minv_lines<P, WAVE>(T, lane, sR);
     const double scale = dt * G.jinv;
     team_for<NB>(lane, [&](int t) {
         const double sc = T.minv_diag ? scale * minv_diag_scale<P>(T, t / V) : scale;
@@ -878,8 +916,9 @@ __device__ __forceinline__ double phys(const KGeom& G, int d, int ie, double xi)
 }
 
 template <int P>
-__global__ __launch_bounds__(WAVE) void init_state_kernel(double time, double* U, KTables T, KGeom G,
-                                                          KIC ic) {
+__global__ __launch_bounds__(WAVE) void init_state_kernel(double time, double* U, const KTables* __restrict__ Tp,
+                                                          KGeom G, KIC ic) {
+    const KTables& T = *Tp;
     using Dm = Dims<P>;
     constexpr int N = Dm::N, NN = Dm::NN, NNN = Dm::NNN, Q = Dm::Q1;
     constexpr int NB = NNN * V;
@@ -936,7 +975,7 @@ __global__ __launch_bounds__(WAVE) void init_state_kernel(double time, double* U
         });
         sync();
     }
-    if (!T.minv_diag) minv_lines<P>(T, lane, rhs);
+    if (!T.minv_diag) minv_lines<P, WAVE>(T, lane, rhs);
     const size_t base = (size_t)e * NB;
     team_for<NB>(lane, [&](int t) {
         const double sc = T.minv_diag ? G.jinv * minv_diag_scale<P>(T, t / V) : G.jinv;
@@ -947,7 +986,8 @@ __global__ __launch_bounds__(WAVE) void init_state_kernel(double time, double* U
 // mode 0: squared L2 error per variable; mode 1: integrals
 template <int P>
 __global__ __launch_bounds__(WAVE) void eval_kernel(int mode, double time, const double* U, double* out,
-                                                    KTables T, KGeom G, KIC ic) {
+                                                    const KTables* __restrict__ Tp, KGeom G, KIC ic) {
+    const KTables& T = *Tp;
     using Dm = Dims<P>;
     constexpr int N = Dm::N, NNN = Dm::NNN, Q = Dm::Q1;
     constexpr int NB = NNN * V;
@@ -1005,7 +1045,8 @@ __global__ __launch_bounds__(WAVE) void eval_kernel(int mode, double time, const
 }
 
 template <int P>
-__global__ __launch_bounds__(WAVE) void wavespeed_kernel(const double* U, double* out, KTables T, KGeom G) {
+__global__ __launch_bounds__(WAVE) void wavespeed_kernel(const double* U, double* out, const KTables* __restrict__ Tp, KGeom G) {
+    const KTables& T = *Tp;
     using Dm = Dims<P>;
     constexpr int N = Dm::N, NNN = Dm::NNN, M = Dm::M;
     constexpr int NB = NNN * V;
@@ -1048,8 +1089,8 @@ __global__ __launch_bounds__(WAVE) void wavespeed_kernel(const double* U, double
 // Dispatch
 // ---------------------------------------------------------------------------
 template <int P>
-hipError_t stage_p(int flux, const StageArgs& a, const KTables& t, const KGeom& g, hipStream_t st) {
-    dim3 grid(g.E), block(WAVE);
+hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& g, hipStream_t st) {
+    dim3 grid(g.E), block(TEAM);
     switch (flux) {
     case 0: hipLaunchKernelGGL((stage_kernel<P, 0>), grid, block, 0, st, a, t, g); break;
     case 1: hipLaunchKernelGGL((stage_kernel<P, 1>), grid, block, 0, st, a, t, g); break;
@@ -1070,31 +1111,31 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables& t, const KGeom& 
     default: return hipErrorInvalidValue;              \
     }
 
-hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables& t, const KGeom& g,
+hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, const KGeom& g,
                         hipStream_t st) {
     DG_DISPATCH_P(P, return stage_p<PP>(flux, a, t, g, st));
     return hipSuccess;
 }
 
-hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables& t,
+hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,
                            const KGeom& g, hipStream_t st) {
     DG_DISPATCH_P(P, hipLaunchKernelGGL((inv_mass_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, dt, R, dU, t, g));
     return hipGetLastError();
 }
 
-hipError_t launch_init_state(int P, double time, double* U, const KTables& t, const KGeom& g,
+hipError_t launch_init_state(int P, double time, double* U, const KTables* t, const KGeom& g,
                              const KIC& ic, hipStream_t st) {
     DG_DISPATCH_P(P, hipLaunchKernelGGL((init_state_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, time, U, t, g, ic));
     return hipGetLastError();
 }
 
-hipError_t launch_eval(int P, int mode, double time, const double* U, double* out, const KTables& t,
+hipError_t launch_eval(int P, int mode, double time, const double* U, double* out, const KTables* t,
                        const KGeom& g, const KIC& ic, hipStream_t st) {
     DG_DISPATCH_P(P, hipLaunchKernelGGL((eval_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, mode, time, U, out, t, g, ic));
     return hipGetLastError();
 }
 
-hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables& t, const KGeom& g,
+hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables* t, const KGeom& g,
                             hipStream_t st) {
     DG_DISPATCH_P(P, hipLaunchKernelGGL((wavespeed_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, U, out, t, g));
     return hipGetLastError();

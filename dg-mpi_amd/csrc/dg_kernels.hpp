@@ -9,7 +9,9 @@ constexpr int PMAX = 5;            // highest order with a compiled kernel
 constexpr int NVAR = 5;            // rho, rho u, rho v, rho w, rho E
 
 // Reference-element operators of one case (1D factors of the tensor bases).
-// Passed by value as kernel arguments (read through the scalar cache).
+// Kept in device memory (one copy per solver) and passed to the kernels by
+// pointer: uniform reads go through the scalar cache, and lane-dependent
+// reads are plain cached loads instead of a per-lane copy of the struct.
 struct KTables {
     double B[PMAX + 2][PMAX + 1];   // phi_i(x_a), a: (p+2)-point GL rule
     double D[PMAX + 2][PMAX + 1];   // phi_i'(x_a)
@@ -64,18 +66,19 @@ struct StageArgs {
 
 // Launchers; P in 1..PMAX, flux 0 LLF / 1 HLLC / 2 Roe.  Return the launch
 // status (hipGetLastError after the launch).
-hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables& t, const KGeom& g,
+// `t` points to the case's KTables in device memory.
+hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, const KGeom& g,
                         hipStream_t st);
-hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables& t,
+hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,
                            const KGeom& g, hipStream_t st);
-hipError_t launch_init_state(int P, double time, double* U, const KTables& t, const KGeom& g,
+hipError_t launch_init_state(int P, double time, double* U, const KTables* t, const KGeom& g,
                              const KIC& ic, hipStream_t st);
 // mode 0: per-element squared L2 error vs exact solution at time t -> out[E][5]
 // mode 1: per-element integrals (rho, rho u, rho v, rho w, rho E, KE) -> out[E][6]
 hipError_t launch_eval(int P, int mode, double time, const double* U, double* out,
-                       const KTables& t, const KGeom& g, const KIC& ic, hipStream_t st);
+                       const KTables* t, const KGeom& g, const KIC& ic, hipStream_t st);
 // per-element max(|u| + c) at the volume points -> out[E]
-hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables& t, const KGeom& g,
+hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables* t, const KGeom& g,
                             hipStream_t st);
 
 }  // namespace dg

@@ -32,6 +32,7 @@ struct dg_solver {
     double* Ub = nullptr;
     double* acc = nullptr;             // RK accumulator
     double* work = nullptr;            // per-element reduction output
+    dg::KTables* dtab = nullptr;       // device copy of tab
     std::string err;
 };
 
@@ -105,7 +106,7 @@ double csum(const std::vector<double>& x, size_t n, size_t stride, size_t off) {
 }
 
 double max_wavespeed(dg_solver* s, const double* U) {
-    hip_check(dg::launch_wavespeed(s->P, U, s->work, s->tab, s->geo, s->stream), "wavespeed kernel");
+    hip_check(dg::launch_wavespeed(s->P, U, s->work, s->dtab, s->geo, s->stream), "wavespeed kernel");
     std::vector<double> h(s->geo.E);
     hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
               "hipMemcpy");
@@ -130,7 +131,7 @@ void step_once(dg_solver* s, double dt) {
         // the new state goes to Ua and is copied back (the state pointer
         // handed out by dg_state_device_ptr stays valid)
         a.Ust = s->U0; a.U0 = s->U0; a.Uout = s->Ua; a.mode = MODE_FE;
-        hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+        hip_check(launch_stage(s->P, flux, a, s->dtab, s->geo, st), "stage kernel");
         hip_check(hipMemcpyAsync(s->U0, s->Ua, s->ncoef * sizeof(double), hipMemcpyDeviceToDevice, st),
                   "hipMemcpyAsync");
         s->time += dt;
@@ -139,15 +140,15 @@ void step_once(dg_solver* s, double dt) {
     // RK4 (src/numerics/timestepping/stepper.cpp:71-142), fused stages
     a.U0 = s->U0; a.acc = s->acc;
     a.Ust = s->U0; a.Uout = s->Ua; a.alpha = 0.5; a.mode = MODE_RK_FIRST;
-    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+    hip_check(launch_stage(s->P, flux, a, s->dtab, s->geo, st), "stage kernel");
     s->time += dt / 2.0;
     a.Ust = s->Ua; a.Uout = s->Ub; a.alpha = 0.5; a.mode = MODE_RK_MID;
-    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+    hip_check(launch_stage(s->P, flux, a, s->dtab, s->geo, st), "stage kernel");
     a.Ust = s->Ub; a.Uout = s->Ua; a.alpha = 1.0; a.mode = MODE_RK_MID;
-    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+    hip_check(launch_stage(s->P, flux, a, s->dtab, s->geo, st), "stage kernel");
     s->time += dt / 2.0;
     a.Ust = s->Ua; a.Uout = s->U0; a.mode = MODE_RK_LAST;
-    hip_check(launch_stage(s->P, flux, a, s->tab, s->geo, st), "stage kernel");
+    hip_check(launch_stage(s->P, flux, a, s->dtab, s->geo, st), "stage kernel");
 }
 
 void free_all(dg_solver* s) {
@@ -155,6 +156,8 @@ void free_all(dg_solver* s) {
     (void)hipSetDevice(s->device);
     for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work})
         if (p) (void)hipFree(p);
+    if (s->dtab) (void)hipFree(s->dtab);
+    s->dtab = nullptr;
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
     s->U0 = s->Ua = s->Ub = s->acc = s->work = nullptr;
     s->stream = s->own_stream = nullptr;
@@ -192,6 +195,8 @@ int dg_create(const char* toml_text, dg_solver** out) {
         hip_check(hipMalloc(&s->Ub, bytes), "hipMalloc(Ub)");
         hip_check(hipMalloc(&s->acc, bytes), "hipMalloc(acc)");
         hip_check(hipMalloc(&s->work, (size_t)s->geo.E * 6 * sizeof(double)), "hipMalloc(work)");
+        hip_check(hipMalloc(&s->dtab, sizeof(dg::KTables)), "hipMalloc(tables)");
+        hip_check(hipMemcpy(s->dtab, &s->tab, sizeof(dg::KTables), hipMemcpyHostToDevice), "hipMemcpy(tables)");
         hip_check(hipMemsetAsync(s->U0, 0, bytes, s->stream), "hipMemset");
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     });
@@ -248,7 +253,7 @@ int dg_init_state(dg_solver* s) {
         need(s, "solver");
         set_device(s);
         s->time = 0.0;
-        hip_check(dg::launch_init_state(s->P, s->time, s->U0, s->tab, s->geo, s->ic, s->stream), "init kernel");
+        hip_check(dg::launch_init_state(s->P, s->time, s->U0, s->dtab, s->geo, s->ic, s->stream), "init kernel");
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
         if (s->cfg.timestep <= 0.0) {
             // CFL rule (new): dt = cfl * min(h) / max(|u| + c) of the initial state
@@ -321,7 +326,7 @@ int dg_residual(dg_solver* s, const double* U_dev, double* R_dev, void* stream) 
         a.Ust = U_dev;
         a.R = R_dev;
         a.mode = dg::MODE_RESIDUAL;
-        hip_check(dg::launch_stage(s->P, (int)s->cfg.flux, a, s->tab, s->geo, pick_stream(s, stream)),
+        hip_check(dg::launch_stage(s->P, (int)s->cfg.flux, a, s->dtab, s->geo, pick_stream(s, stream)),
                   "stage kernel");
     });
 }
@@ -332,7 +337,7 @@ int dg_apply_inv_mass(dg_solver* s, double dt, const double* R_dev, double* dU_d
         need(R_dev, "R_dev");
         need(dU_dev, "dU_dev");
         set_device(s);
-        hip_check(dg::launch_inv_mass(s->P, dt, R_dev, dU_dev, s->tab, s->geo, pick_stream(s, stream)),
+        hip_check(dg::launch_inv_mass(s->P, dt, R_dev, dU_dev, s->dtab, s->geo, pick_stream(s, stream)),
                   "inverse mass kernel");
     });
 }
@@ -365,7 +370,7 @@ int dg_l2_error(dg_solver* s, double t, double err[5]) {
         need(s, "solver");
         need(err, "err");
         set_device(s);
-        hip_check(dg::launch_eval(s->P, 0, t, s->U0, s->work, s->tab, s->geo, s->ic, s->stream), "eval kernel");
+        hip_check(dg::launch_eval(s->P, 0, t, s->U0, s->work, s->dtab, s->geo, s->ic, s->stream), "eval kernel");
         std::vector<double> h((size_t)s->geo.E * 6);
         hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
                   "hipMemcpy");
@@ -386,7 +391,7 @@ int dg_integrals(dg_solver* s, double out[6]) {
         need(s, "solver");
         need(out, "out");
         set_device(s);
-        hip_check(dg::launch_eval(s->P, 1, s->time, s->U0, s->work, s->tab, s->geo, s->ic, s->stream), "eval kernel");
+        hip_check(dg::launch_eval(s->P, 1, s->time, s->U0, s->work, s->dtab, s->geo, s->ic, s->stream), "eval kernel");
         std::vector<double> h((size_t)s->geo.E * 6);
         hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
                   "hipMemcpy");

@@ -20,7 +20,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdg_amd.so")
+# DG_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("DG_AMD_LIB") or os.path.join(PKG_ROOT, "lib", "libdg_amd.so")
 
 DG_LAYOUT_RIGHT = 0
 DG_LAYOUT_LEFT = 1

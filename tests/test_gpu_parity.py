@@ -199,3 +199,53 @@ def test_cfl_timestep_and_conservation_of_integrals():
     I1 = g.integrals()
     # mass, momentum and energy are conserved to round-off
     assert np.all(np.abs(I1[:5] - I0[:5]) < 1e-12 * np.abs(I0[:5]).max())
+
+
+def test_dg_run_and_dg_post_on_the_reference_example(tmp_path, golden_dir):
+    """dg_run / dg_post (counterparts of src/exec/main.cpp and
+    src/post/main.cpp) on the case of examples/euler/3D/isentropic_vortex/
+    input.toml with the HDF5 mesh replaced by the periodic-box keys."""
+    import re
+    import subprocess
+    gd = json.load(open(os.path.join(golden_dir, "vortex3d_regression.json")))
+    case = """[Mesh]
+type = "PeriodicBox"
+nelem = [4, 4, 4]
+lower = [0.0, 0.0, 0.0]
+upper = [1.0, 1.0, 1.0]
+
+[Numerics]
+order = 3
+basis = "LagrangeHex"
+
+[Stepper]
+start = 0.0
+end = 1.0
+timestep = 1e-3
+type = "RK4"
+
+[Physics]
+name = "Euler"
+dim = 3
+gamma = 1.4
+GasConstant = 1.0
+convective_flux_fcn = "HLLC"
+
+[InitialCondition]
+name = "IsentropicVortex"
+data = [1.0, 1.0, 1.0, 1.0, 5.0]
+"""
+    (tmp_path / "input.toml").write_text(case)
+    bindir = os.path.join(os.path.dirname(dg_amd.PKG_ROOT), "dg-mpi_amd", "bin")
+    r = subprocess.run([os.path.join(bindir, "dg_run")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Number of iterations = 1000" in r.stdout
+    r = subprocess.run([os.path.join(bindir, "dg_post")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    errs = [float(v) for v in re.findall(r"norm_total_error\(\d\)=([-+0-9.eE]+)", r.stdout)]
+    ref = [float(v) for v in gd["l2_errors"]]
+    assert len(errs) == 5
+    for v in (0, 1, 3, 4):
+        assert abs(errs[v] - ref[v]) < 1e-10 * ref[v]
