@@ -2,11 +2,13 @@
 // Reads a TOML case (-input, default input.toml), projects the initial
 // condition, runs Solver::solve and writes the final coefficients.
 //
-//   dg_run [-input case.toml] [-output data.dgs]
+//   dg_run [-input case.toml] [-output data.h5]
 //
-// The output is a raw file: 8-byte magic "DGAMDSOL", int64 E, int32 nb,
-// int32 ns, double time, int32 stored_layout (1 = row major, the reference's
-// "Stored Layout" flag, src/io/writer.cpp:72), then E*nb*ns doubles.
+// Output: the reference's data.h5 layout (src/io/writer.cpp:10-98) when the
+// tools are built with HDF5, and for any output name ending in ".dgs" (or
+// without HDF5) a raw file: 8-byte magic "DGAMDSOL", int64 E, int32 nb,
+// int32 ns, double time, int32 stored_layout (1 = row major, the
+// reference's "Stored Layout" flag), then E*nb*ns doubles.
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -17,6 +19,9 @@
 #include <vector>
 
 #include "dg_amd.h"
+#ifdef DG_WITH_HDF5
+#include "dg_amd_h5.h"
+#endif
 
 static const char* opt(int argc, char** argv, const char* key, const char* dflt) {
     for (int i = 1; i + 1 < argc; ++i)
@@ -24,9 +29,42 @@ static const char* opt(int argc, char** argv, const char* key, const char* dflt)
     return dflt;
 }
 
+static bool ends_with(const std::string& s, const char* suf) {
+    const size_t n = std::strlen(suf);
+    return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
+}
+
+static int write_raw(dg_solver* s, const dg_info& info, const std::string& out) {
+    std::vector<double> U((size_t)info.num_elems * info.num_basis * info.num_state_vars);
+    if (dg_get_state(s, U.data(), DG_LAYOUT_RIGHT) != DG_OK) {
+        std::fprintf(stderr, "error: %s\n", dg_last_error(s));
+        return 1;
+    }
+    std::ofstream o(out, std::ios::binary);
+    const char magic[8] = {'D', 'G', 'A', 'M', 'D', 'S', 'O', 'L'};
+    const int64_t E = info.num_elems;
+    const int32_t nb = info.num_basis, ns = info.num_state_vars, layout = 1;
+    o.write(magic, 8);
+    o.write((const char*)&E, 8);
+    o.write((const char*)&nb, 4);
+    o.write((const char*)&ns, 4);
+    o.write((const char*)&info.time, 8);
+    o.write((const char*)&layout, 4);
+    o.write((const char*)U.data(), U.size() * sizeof(double));
+    if (!o) {
+        std::fprintf(stderr, "cannot write %s\n", out.c_str());
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     const std::string in = opt(argc, argv, "-input", "input.toml");
+#ifdef DG_WITH_HDF5
+    const std::string out = opt(argc, argv, "-output", "data.h5");
+#else
     const std::string out = opt(argc, argv, "-output", "data.dgs");
+#endif
     std::ifstream f(in);
     if (!f) { std::fprintf(stderr, "cannot open %s\n", in.c_str()); return 2; }
     std::stringstream ss;
@@ -49,20 +87,17 @@ int main(int argc, char** argv) {
     dg_get_info(s, &info);
     std::printf("Time to Solution: %.6f s (%.3f ms per step), time = %.17g\n", sec,
                 info.num_iterations ? 1e3 * sec / info.num_iterations : 0.0, info.time);
-    std::vector<double> U((size_t)info.num_elems * info.num_basis * info.num_state_vars);
-    dg_get_state(s, U.data(), DG_LAYOUT_RIGHT);
-    std::ofstream o(out, std::ios::binary);
-    const char magic[8] = {'D', 'G', 'A', 'M', 'D', 'S', 'O', 'L'};
-    const int64_t E = info.num_elems;
-    const int32_t nb = info.num_basis, ns = info.num_state_vars, layout = 1;
-    o.write(magic, 8);
-    o.write((const char*)&E, 8);
-    o.write((const char*)&nb, 4);
-    o.write((const char*)&ns, 4);
-    o.write((const char*)&info.time, 8);
-    o.write((const char*)&layout, 4);
-    o.write((const char*)U.data(), U.size() * sizeof(double));
-    std::printf("Solution written to %s\n", out.c_str());
+    int wrc = 0;
+#ifdef DG_WITH_HDF5
+    if (!ends_with(out, ".dgs")) {
+        if (dg_write_solution(s, out.c_str()) != DG_OK) {
+            std::fprintf(stderr, "error: %s\n", dg_h5_last_error());
+            wrc = 1;
+        }
+    } else
+#endif
+        wrc = write_raw(s, info, out);
+    if (wrc == 0) std::printf("Solution written to %s\n", out.c_str());
     dg_destroy(s);
-    return 0;
+    return wrc;
 }

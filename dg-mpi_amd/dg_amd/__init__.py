@@ -213,6 +213,17 @@ class Solver:
         """Launch on a caller-owned HIP stream handle (int), None = own."""
         self._check(lib().dg_set_stream(self._h, C.c_void_p(stream) if stream else None))
 
+    # -- data.h5 files (libdg_amd_h5.so) --
+    def write(self, path="data.h5"):
+        """Write the state and time in the reference's data.h5 layout."""
+        from . import h5
+        h5.write_solution(self, path)
+
+    def read_in_coefficients(self, path="data.h5", set_time=True):
+        """Load a data.h5 file (Solver::read_in_coefficients)."""
+        from . import h5
+        h5.read_solution(self, path, set_time)
+
 
 def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(-5., -5., -5.),
               upper=(5., 5., 5.), stepper="RK4", timestep=None, cfl=None, end=1.0, gamma=1.4,
