@@ -1,19 +1,23 @@
 // HIP kernels (gfx950) of the DG residual + RK path: 3D Euler on periodic
 // hexahedral boxes, FP64.
 //
-// Design (one 64-lane wavefront per element, all exchanges in LDS):
+// Design (a team of 128 threads -- two wavefronts -- per element, all
+// exchanges in LDS):
 //   * The whole RK stage is one launch: face traces of the element and of
 //     its six neighbours, the Riemann fluxes, the lift, the sum-factorised
 //     volume integral, the inverse mass matrix and the RK register update.
 //     The state is read once from HBM per stage (neighbour blocks come back
 //     through L2/MALL); only the RK registers add HBM traffic.
+//   * Persistent workgroups: each walks a list of elements and prefetches
+//     the next element's block, neighbour lines and RK registers into
+//     registers while the current one is computed.
 //   * No atomics: every element recomputes the trace of each neighbour on
 //     the shared face with exactly the arithmetic the neighbour uses for its
 //     own trace, and evaluates the Riemann flux with the same (UL, UR, N), so
 //     both sides see bit-identical fluxes (exact discrete conservation).
 //   * Tensor-product sum factorisation: each contraction step is a set of
-//     (line, variable) tasks; a lane keeps its line in registers, basis
-//     matrices come from the kernel arguments through the scalar cache.
+//     (line, variable) tasks; a thread keeps its line in registers, basis
+//     matrices are read from a device-resident table.
 //
 // Reference semantics restated here (paths relative to the reference root):
 //   residual            src/solver/base.cpp:401-561
@@ -46,6 +50,17 @@ struct Dims {
     static constexpr int N = P + 1, M = P + 2;
     static constexpr int NN = N * N, MM = M * M, NNN = N * N * N, MMM = M * M * M;
     static constexpr int Q1 = 2 * P + 2;
+};
+
+// The few table entries that the stage kernel indexes with per-thread
+// indices, copied to LDS once per workgroup.  Reading them from global
+// memory would issue vector loads, and waiting for those would also wait
+// for every prefetch load issued before them.
+struct LaneTab {
+    double w[PMAX + 2];      // GL weights of the (p+2)-point rule
+    double phim[PMAX + 1];   // phi_i(-1)
+    double phip[PMAX + 1];   // phi_i(+1)
+    double mdiag[PMAX + 1];  // diagonal of the 1D inverse mass matrix
 };
 
 template <int T, int TS = WAVE, class F>
@@ -434,9 +449,8 @@ __device__ __forceinline__ int bidx_d(int d, int r, int t1, int t2) {
 // same trace and the same flux.
 // ---------------------------------------------------------------------------
 template <int P, int FLUX, int TS>
-__device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T, const KGeom& G,
-                                           const int* ec, int lane, const double* sU, double* sR,
-                                           double* bankA, double* bankB) {
+__device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, const KGeom& G, int lane,
+                                           const double* sU, double* sR, double* bankA, double* bankB) {
     using Dm = Dims<P>;
     using L = StageLDS<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, NNN = Dm::NNN;
@@ -447,22 +461,7 @@ __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T,
     double* l1 = bankB;
     double* l2 = bankA + L::U;
 
-    // 1. normal contractions: neighbour slabs (global loads issued first),
-    //    then own slabs of both sides
-    team_for<6 * NN * V, TS>(lane, [&](int t) {
-        const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
-        const int d = ds >> 1, side = ds & 1;
-        const int off = side ? 1 : -1;
-        const unsigned e = elem_at(ec[0] + (d == 0 ? off : 0), ec[1] + (d == 1 ? off : 0),
-                                   ec[2] + (d == 2 ? off : 0), G);
-        const double* Un = A.Ust + (size_t)e * (NNN * V);
-        // neighbour across the - face contributes its + trace and vice versa
-        const double* phi = side == 0 ? T.phip : T.phim;
-        double acc = 0.;
-#pragma unroll
-        for (int r = 0; r < N; ++r) acc = fma(phi[r], Un[bidx_d<N>(d, r, t1, t2) * V + s], acc);
-        slab[((d * 4 + 2 + side) * NN + t2 * N + t1) * V + s] = acc;
-    });
+    // 1. own slabs of both sides (the neighbour slabs are already in `slab`)
     team_for<3 * NN * V, TS>(lane, [&](int t) {
         const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, d = u2 / N;
         double um = 0., up = 0.;
@@ -524,7 +523,7 @@ __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T,
         if (d == 0) riemann<FLUX, 0>(G.gamma, UL, UR, G.farea[0], F);
         else if (d == 1) riemann<FLUX, 1>(G.gamma, UL, UR, G.farea[1], F);
         else riemann<FLUX, 2>(G.gamma, UL, UR, G.farea[2], F);
-        const double wq = T.w[a1] * T.w[a2];
+        const double wq = LT.w[a1] * LT.w[a2];
         double* o = fh + (((d * 2 + side) * M + a2) * M + a1) * V;
 #pragma unroll
         for (int s = 0; s < V; ++s) {
@@ -567,11 +566,11 @@ __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T,
         const int s = t % V, b = t / V, i = b % N, j = (b / N) % N, k = b / NN;
         double acc = sR[t];
         // d = 0: r = i, (t1, t2) = (j, k)
-        acc += T.phim[i] * l2[((0 * N + k) * N + j) * V + s] + T.phip[i] * l2[((1 * N + k) * N + j) * V + s];
+        acc += LT.phim[i] * l2[((0 * N + k) * N + j) * V + s] + LT.phip[i] * l2[((1 * N + k) * N + j) * V + s];
         // d = 1: r = j, (t1, t2) = (i, k)
-        acc += T.phim[j] * l2[((2 * N + k) * N + i) * V + s] + T.phip[j] * l2[((3 * N + k) * N + i) * V + s];
+        acc += LT.phim[j] * l2[((2 * N + k) * N + i) * V + s] + LT.phip[j] * l2[((3 * N + k) * N + i) * V + s];
         // d = 2: r = k, (t1, t2) = (i, j)
-        acc += T.phim[k] * l2[((4 * N + j) * N + i) * V + s] + T.phip[k] * l2[((5 * N + j) * N + i) * V + s];
+        acc += LT.phim[k] * l2[((4 * N + j) * N + i) * V + s] + LT.phip[k] * l2[((5 * N + j) * N + i) * V + s];
         sR[t] = acc;
     });
     sync();
@@ -581,7 +580,7 @@ __device__ __forceinline__ void face_phase(const StageArgs& A, const KTables& T,
 // Volume phase: sum-factorised interpolation, point fluxes, weak gradient.
 // ---------------------------------------------------------------------------
 template <int P, int TS>
-__device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, int lane,
+__device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT, const KGeom& G, int lane,
                                              double* sR, double* bankA, double* bankB) {
     using Dm = Dims<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, MMM = Dm::MMM;
@@ -662,7 +661,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const KGeom& G, i
     // along z: B for x and y fluxes, D for the z flux
     team_for<MM * V, TS>(lane, [&](int t) {
         const int s = t % V, ba = t / V, a = ba % M, b = ba / M;
-        const double wab = T.w[a] * T.w[b];
+        const double wab = LT.w[a] * LT.w[b];
         double xx[N], yy[N], zz[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) xx[k] = yy[k] = zz[k] = 0.;
@@ -798,59 +797,165 @@ __device__ __forceinline__ double minv_diag_scale(const KTables& T, int b) {
     return T.minv[i][i] * T.minv[j][j] * T.minv[k][k];
 }
 
+// the same product from the LDS copy of the diagonal
+template <int P>
+__device__ __forceinline__ double lane_diag_scale(const LaneTab& LT, int b) {
+    constexpr int N = P + 1;
+    const int i = b % N, j = (b / N) % N, k = b / (N * N);
+    return LT.mdiag[i] * LT.mdiag[j] * LT.mdiag[k];
+}
+
 // ---------------------------------------------------------------------------
-// Fused stage kernel
+// Fused stage kernel: one workgroup per element.
+//
+// All global loads of the element -- its coefficient block, the lines of
+// its six neighbours that feed the neighbour traces, and its RK registers --
+// are issued together into registers at the start.  Workgroups are mapped
+// to elements XCD-contiguously (block_element), so neighbour blocks read by
+// concurrently running workgroups share an XCD's L2.
 // ---------------------------------------------------------------------------
+template <int P, int TS>
+struct StageLoads {
+    using Dm = Dims<P>;
+    static constexpr int N = Dm::N, NN = Dm::NN, NB = Dm::NNN * V;
+    static constexpr int RU = (NB + TS - 1) / TS;          // rounds over an element block
+    static constexpr int NT = 6 * NN * V;                  // neighbour-slab tasks
+    static constexpr int RN = (NT + TS - 1) / TS;
+    double own[RU];
+    double nl[RN][N];
+    double u0[RU], ac[RU];
+
+    __device__ __forceinline__ void load_block(const double* Ust, unsigned e, int lane) {
+        const double* src = Ust + (size_t)e * NB;
+#pragma unroll
+        for (int r = 0; r < RU; ++r) {
+            const int t = r * TS + lane;
+            own[r] = t < NB ? src[t] : 0.;
+        }
+    }
+    __device__ __forceinline__ void load_neighbours(const double* Ust, unsigned e, const KGeom& G, int lane) {
+        int ec[3];
+        elem_ijk(e, G, ec);
+#pragma unroll
+        for (int r = 0; r < RN; ++r) {
+            const int t = r * TS + lane;
+            if (t < NT) {
+                const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
+                const int d = ds >> 1, side = ds & 1;
+                const int off = side ? 1 : -1;
+                const unsigned en = elem_at(ec[0] + (d == 0 ? off : 0), ec[1] + (d == 1 ? off : 0),
+                                            ec[2] + (d == 2 ? off : 0), G);
+                const double* Un = Ust + (size_t)en * NB;
+#pragma unroll
+                for (int q = 0; q < N; ++q) nl[r][q] = Un[bidx_d<N>(d, q, t1, t2) * V + s];
+            }
+        }
+    }
+    __device__ __forceinline__ void load_rk(const StageArgs& A, unsigned e, int lane) {
+        const size_t base = (size_t)e * NB;
+        const bool need_acc = A.mode == MODE_RK_MID || A.mode == MODE_RK_LAST;
+#pragma unroll
+        for (int r = 0; r < RU; ++r) {
+            const int t = r * TS + lane;
+            u0[r] = t < NB ? A.U0[base + t] : 0.;
+            ac[r] = (t < NB && need_acc) ? A.acc[base + t] : 0.;
+        }
+    }
+    __device__ __forceinline__ void clear_rk() {
+#pragma unroll
+        for (int r = 0; r < RU; ++r) u0[r] = ac[r] = 0.;
+    }
+    // neighbour slabs from the prefetched lines: the neighbour across the -
+    // face contributes its + trace and vice versa; same fma chain as the
+    // neighbour's own slab of that face
+    __device__ __forceinline__ void store_neighbour_slabs(const LaneTab& LT, double* slab, int lane) const {
+#pragma unroll
+        for (int r = 0; r < RN; ++r) {
+            const int t = r * TS + lane;
+            if (t < NT) {
+                const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
+                const int d = ds >> 1, side = ds & 1;
+                const double* phi = side == 0 ? LT.phip : LT.phim;
+                double acc = 0.;
+#pragma unroll
+                for (int q = 0; q < N; ++q) acc = fma(phi[q], nl[r][q], acc);
+                slab[((d * 4 + 2 + side) * NN + t2 * N + t1) * V + s] = acc;
+            }
+        }
+    }
+};
+
 template <int P, int FLUX>
 __global__ __launch_bounds__(TEAM) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp, KGeom G) {
     constexpr int TS = TEAM;
     const KTables& T = *Tp;
     using Dm = Dims<P>;
     using L = StageLDS<P>;
-    constexpr int NNN = Dm::NNN;
-    constexpr int NB = NNN * V;
+    using PF = StageLoads<P, TS>;
+    constexpr int NB = Dm::NNN * V;
     __shared__ double lds[L::TOTAL];
+    __shared__ LaneTab LT;
     double* sR = lds;
     double* bankA = lds + L::R;
     double* bankB = bankA + L::A;
     const int lane = threadIdx.x;
     const unsigned e = block_element(G);
-    int ec[3];
-    elem_ijk(e, G, ec);
+    const bool rk = A.mode != MODE_RESIDUAL;
 
-    const double* Ue = A.Ust + (size_t)e * NB;
-    team_for<NB, TS>(lane, [&](int t) {
-        bankA[t] = Ue[t];
-        sR[t] = 0.;
-    });
-    sync();
-    face_phase<P, FLUX, TS>(A, T, G, ec, lane, bankA, sR, bankA, bankB);
-    volume_phase<P, TS>(T, G, lane, sR, bankA, bankB);
-
-    const size_t base = (size_t)e * NB;
-    if (A.mode == MODE_RESIDUAL) {
-        team_for<NB, TS>(lane, [&](int t) { A.R[base + t] = sR[t]; });
-        return;
+    // lane tables first (their loads are the first to return), then every
+    // global load of the element at once: its block, the neighbour lines
+    // and the RK registers
+    if (lane < PMAX + 2) LT.w[lane] = T.w[lane];
+    if (lane < PMAX + 1) {
+        LT.phim[lane] = T.phim[lane];
+        LT.phip[lane] = T.phip[lane];
+        LT.mdiag[lane] = T.minv[lane][lane];
     }
-    double scale = A.dt * G.jinv;
-    if (!T.minv_diag) minv_lines<P, TS>(T, lane, sR);
-    const int mode = A.mode;
-    team_for<NB, TS>(lane, [&](int t) {
-        const double sc = T.minv_diag ? scale * minv_diag_scale<P>(T, t / V) : scale;
-        const double dU = sc * sR[t];
-        const size_t g = base + t;
-        if (mode == MODE_RK_FIRST) {
-            A.Uout[g] = A.U0[g] + A.alpha * dU;
-            A.acc[g] = dU;
-        } else if (mode == MODE_RK_MID) {
-            A.Uout[g] = A.U0[g] + A.alpha * dU;
-            A.acc[g] = A.acc[g] + 2.0 * dU;
-        } else if (mode == MODE_RK_LAST) {
-            A.Uout[g] = A.U0[g] + (1.0 / 6.0) * (A.acc[g] + dU);
-        } else {
-            A.Uout[g] = A.U0[g] + dU;
+    PF pf;
+    pf.load_block(A.Ust, e, lane);
+    pf.load_neighbours(A.Ust, e, G, lane);
+    if (rk) pf.load_rk(A, e, lane);
+    else pf.clear_rk();
+    sync();
+#pragma unroll
+    for (int r = 0; r < PF::RU; ++r) {
+        const int t = r * TS + lane;
+        if (t < NB) {
+            bankA[t] = pf.own[r];
+            sR[t] = 0.;
         }
-    });
+    }
+    pf.store_neighbour_slabs(LT, bankB, lane);
+    sync();
+    face_phase<P, FLUX, TS>(T, LT, G, lane, bankA, sR, bankA, bankB);
+    volume_phase<P, TS>(T, LT, G, lane, sR, bankA, bankB);
+
+    // Epilogue with one store site per array; the value and the target are
+    // picked by selects on the (uniform) mode.  Written as one branch per
+    // mode, the backend has merged the stores into one block whose base
+    // register was left unset on the forward-Euler path.
+    const size_t base = (size_t)e * NB;
+    const int mode = A.mode;
+    if (rk && !T.minv_diag) minv_lines<P, TS>(T, lane, sR);
+    const double scale = A.dt * G.jinv;
+    double* const out = rk ? A.Uout : A.R;
+    const bool wacc = mode == MODE_RK_FIRST || mode == MODE_RK_MID;
+#pragma unroll
+    for (int r = 0; r < PF::RU; ++r) {
+        const int t = r * TS + lane;
+        if (t < NB) {
+            const double rv = sR[t];
+            const double sc = T.minv_diag ? scale * lane_diag_scale<P>(LT, t / V) : scale;
+            const double dU = sc * rv;
+            const double u0 = pf.u0[r], ac = pf.ac[r];
+            const double n_rk = u0 + A.alpha * dU;                 // U0 + alpha dU
+            const double n_last = u0 + (1.0 / 6.0) * (ac + dU);    // U0 + (acc + dU) / 6
+            const double n_fe = u0 + dU;                           // U0 + dU
+            const double nv = !rk ? rv : (mode == MODE_RK_LAST ? n_last : (mode == MODE_FE ? n_fe : n_rk));
+            out[base + t] = nv;
+            if (wacc) A.acc[base + t] = mode == MODE_RK_FIRST ? dU : ac + 2.0 * dU;
+        }
+    }
 }
 
 // dU = dt M^-1 R
