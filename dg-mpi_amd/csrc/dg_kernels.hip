@@ -1,23 +1,23 @@
 // HIP kernels (gfx950) of the DG residual + RK path: 3D Euler on periodic
 // hexahedral boxes, FP64.
 //
-// Design (a team of 128 threads -- two wavefronts -- per element, all
-// exchanges in LDS):
+// Design (a team of 2-4 wavefronts per element, all exchanges in LDS):
 //   * The whole RK stage is one launch: face traces of the element and of
 //     its six neighbours, the Riemann fluxes, the lift, the sum-factorised
 //     volume integral, the inverse mass matrix and the RK register update.
 //     The state is read once from HBM per stage (neighbour blocks come back
 //     through L2/MALL); only the RK registers add HBM traffic.
-//   * Persistent workgroups: each walks a list of elements and prefetches
-//     the next element's block, neighbour lines and RK registers into
-//     registers while the current one is computed.
+//   * All global loads of an element (its block, the neighbour lines that
+//     feed the neighbour traces, the RK registers) are issued together at
+//     the start of the workgroup.
 //   * No atomics: every element recomputes the trace of each neighbour on
 //     the shared face with exactly the arithmetic the neighbour uses for its
 //     own trace, and evaluates the Riemann flux with the same (UL, UR, N), so
 //     both sides see bit-identical fluxes (exact discrete conservation).
 //   * Tensor-product sum factorisation: each contraction step is a set of
 //     (line, variable) tasks; a thread keeps its line in registers, basis
-//     matrices are read from a device-resident table.
+//     matrices are read from a device-resident table through the scalar
+//     cache, per-thread-indexed table entries from an LDS copy.
 //
 // Reference semantics restated here (paths relative to the reference root):
 //   residual            src/solver/base.cpp:401-561
@@ -39,11 +39,18 @@ namespace {
 
 constexpr int V = NVAR;
 constexpr int WAVE = 64;
-// threads that cooperate on one element in the fused stage kernel
-#ifndef DG_STAGE_TEAM
-#define DG_STAGE_TEAM 128
+// Threads that cooperate on one element in the fused stage kernel: four
+// wavefronts from p = 3 on (measured at 64^3 p = 3: 64 threads 3.47 ms per
+// stage, 128: 2.48, 192: 2.26, 256: 2.12, 320: 3.33), two below, where an
+// element has fewer (line, variable) tasks.  DG_STAGE_TEAM overrides.
+template <int P>
+constexpr int stage_team() {
+#ifdef DG_STAGE_TEAM
+    return DG_STAGE_TEAM;
+#else
+    return P <= 2 ? 128 : 256;
 #endif
-constexpr int TEAM = DG_STAGE_TEAM;
+}
 
 template <int P>
 struct Dims {
@@ -91,6 +98,35 @@ __device__ __forceinline__ double pressure_r(double gam, const double* U, double
     return (gam - 1.0) * (U[4] - 0.5 * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * r1));
 }
 
+// Reciprocal and square root of the flux computations: hardware estimates
+// refined by Newton-Raphson (reciprocal: two steps r <- r + r (1 - x r))
+// and by Goldschmidt's iteration with two residual corrections (square
+// root, from the reciprocal square-root estimate).  For the positive normal
+// arguments they receive here they are within an ulp or two of the
+// correctly rounded results, at about half the instructions of the exact
+// IEEE expansions.  Both sides of a face evaluate the same code, so the
+// flux stays bit-identical between them.
+__device__ __forceinline__ double frcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+__device__ __forceinline__ double fsqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    // zero and negative arguments (not physical here) as sqrt has them
+    return x > 0.0 ? g : (x == 0.0 ? 0.0 : __builtin_nan(""));
+}
+
 // src/physics/euler/functions.cpp:25-40
 __device__ __forceinline__ double max_wave_speed(double gam, const double* U) {
     const double rho1 = 1. / U[0];
@@ -116,13 +152,13 @@ __device__ __forceinline__ void flux_axis(const double* U, double r1, double P, 
 template <int D>
 __device__ __forceinline__ void riemann_llf(double gam, const double* UL, const double* UR,
                                             double nmag, double* F) {
-    const double r1L = 1. / UL[0], r1R = 1. / UR[0];
+    const double r1L = frcp(UL[0]), r1R = frcp(UR[0]);
     const double PL = pressure_r(gam, UL, r1L), PR = pressure_r(gam, UR, r1R);
     double FL[V], FR[V];
     flux_axis<D>(UL, r1L, PL, FL);
     flux_axis<D>(UR, r1R, PR, FR);
-    double a = sqrt(UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) * r1L + sqrt(gam * PL * r1L);
-    const double aR = sqrt(UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]) * r1R + sqrt(gam * PR * r1R);
+    double a = fsqrt(UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) * r1L + fsqrt(gam * PL * r1L);
+    const double aR = fsqrt(UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]) * r1R + fsqrt(gam * PR * r1R);
     if (aR > a) a = aR;
 #pragma unroll
     for (int s = 0; s < V; ++s) F[s] = nmag * (0.5 * (FL[s] + FR[s]) - 0.5 * a * (UR[s] - UL[s]));
@@ -133,28 +169,28 @@ template <int D>
 __device__ __forceinline__ void riemann_hllc(double gam, const double* UL, const double* UR,
                                              double nmag, double* F) {
     const double djac = nmag;
-    const double rhoL = UL[0], rhoL1 = 1. / rhoL;
+    const double rhoL = UL[0], rhoL1 = frcp(rhoL);
     const double unL = UL[1 + D] * rhoL1;
     const double PL = pressure_r(gam, UL, rhoL1);
-    const double aL = sqrt(gam * PL * rhoL1);
-    const double rhoR = UR[0], rhoR1 = 1. / rhoR;
+    const double aL = fsqrt(gam * PL * rhoL1);
+    const double rhoR = UR[0], rhoR1 = frcp(rhoR);
     const double unR = UR[1 + D] * rhoR1;
     const double PR = pressure_r(gam, UR, rhoR1);
-    const double aR = sqrt(gam * PR * rhoR1);
+    const double aR = fsqrt(gam * PR * rhoR1);
     const double rAvg = 0.5 * (rhoL + rhoR);
     const double aAvg = 0.5 * (aL + aR);
     const double pStar = fmax(0., 0.5 * (PL + PR - (unR - unL) * rAvg * aAvg));
     const double gq = (gam + 1.) / (2. * gam);
     double qL = 1.0;
-    const double xL = pStar / PL;
-    if (xL > 1.) qL = sqrt(1. + gq * (xL - 1.));
+    const double xL = pStar * frcp(PL);
+    if (xL > 1.) qL = fsqrt(1. + gq * (xL - 1.));
     double qR = 1.0;
-    const double xR = pStar / PR;
-    if (xR > 1.) qR = sqrt(1. + gq * (xR - 1.));
+    const double xR = pStar * frcp(PR);
+    if (xR > 1.) qR = fsqrt(1. + gq * (xR - 1.));
     const double sL = unL - aL * qL;
     const double sR = unR + aR * qR;
-    const double sStar = (PR - PL + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR)) /
-                         (rhoL * (sL - unL) - rhoR * (sR - unR));
+    const double sStar = (PR - PL + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR)) *
+                         frcp(rhoL * (sL - unL) - rhoR * (sR - unR));
     if (sL >= 0.) {
         F[0] = djac * rhoL * unL;
 #pragma unroll
@@ -172,9 +208,9 @@ __device__ __forceinline__ void riemann_hllc(double gam, const double* UL, const
         }
         F[4] = djac * (UR[4] + PR) * unR;
     } else if ((sL < 0.) && (sStar >= 0.)) {
-        const double c = (sL - unL) / (sL - sStar);
+        const double c = (sL - unL) * frcp(sL - sStar);
         const double f1 = sStar - unL;
-        const double f2 = sStar + PL / (rhoL * (sL - unL));
+        const double f2 = sStar + PL * frcp(rhoL * (sL - unL));
         F[0] = djac * (rhoL * unL + rhoL * sL * (c - 1.));
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -184,9 +220,9 @@ __device__ __forceinline__ void riemann_hllc(double gam, const double* UL, const
         }
         F[4] = djac * ((UL[4] + PL) * unL + sL * (UL[4] * (c - 1.) + rhoL * c * f1 * f2));
     } else if ((sStar < 0.) && (sR > 0.)) {
-        const double c = (sR - unR) / (sR - sStar);
+        const double c = (sR - unR) * frcp(sR - sStar);
         const double f1 = sStar - unR;
-        const double f2 = sStar + PR / (rhoR * (sR - unR));
+        const double f2 = sStar + PR * frcp(rhoR * (sR - unR));
         F[0] = djac * (rhoR * unR + rhoR * sR * (c - 1.));
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -217,7 +253,7 @@ template <int D>
 __device__ __forceinline__ void riemann_roe(double gam, const double* UL, const double* UR,
                                             double nmag, double* F) {
     const double rL = UL[0], rR = UR[0];
-    const double r1L = 1. / rL, r1R = 1. / rR;
+    const double r1L = frcp(rL), r1R = frcp(rR);
     double uL[3], uR[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -228,22 +264,22 @@ __device__ __forceinline__ void riemann_roe(double gam, const double* UL, const 
     const double pL = gm1 * (UL[4] - 0.5 * (UL[1] * uL[0] + UL[2] * uL[1] + UL[3] * uL[2]));
     const double pR = gm1 * (UR[4] - 0.5 * (UR[1] * uR[0] + UR[2] * uR[1] + UR[3] * uR[2]));
     const double HL = (UL[4] + pL) * r1L, HR = (UR[4] + pR) * r1R;
-    const double sL = sqrt(rL), sR = sqrt(rR), isum = 1. / (sL + sR);
+    const double sL = fsqrt(rL), sR = fsqrt(rR), isum = frcp(sL + sR);
     double ut[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) ut[d] = (sL * uL[d] + sR * uR[d]) * isum;
     const double Ht = (sL * HL + sR * HR) * isum;
     const double q2 = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
     const double c2 = gm1 * (Ht - 0.5 * q2);
-    const double ct = sqrt(c2);
-    const double ic2 = 1. / c2;
+    const double ct = fsqrt(c2);
+    const double ic2 = frcp(c2);
     const double rt = sL * sR;
     const double unt = ut[D], unL = uL[D], unR = uR[D];
     const double drho = rR - rL, dp = pR - pL, dun = unR - unL;
     const double a1 = (dp - rt * ct * dun) * (0.5 * ic2);
     const double a5 = (dp + rt * ct * dun) * (0.5 * ic2);
     const double a2 = drho - dp * ic2;
-    const double cL = sqrt(gam * pL * r1L), cR = sqrt(gam * pR * r1R);
+    const double cL = fsqrt(gam * pL * r1L), cR = fsqrt(gam * pR * r1R);
     const double l1 = entropy_fix(unt - ct, unL - cL, unR - cR);
     const double l2 = fabs(unt);
     const double l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
@@ -371,11 +407,20 @@ __device__ void ic_eval(const KIC& ic, const KGeom& g, const double* x, double t
 
 __device__ __forceinline__ unsigned block_element(const KGeom& g) {
     const unsigned w = blockIdx.x;
-    if (!g.xcd_remap) return w;
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous slab of elements so neighbour blocks hit the same L2.
-    const unsigned per = g.E >> 3;
-    return (w & 7u) * per + (w >> 3);
+    // contiguous range of the traversal so neighbour blocks hit its L2.
+    const unsigned p = g.xcd_remap ? (w & 7u) * (g.E >> 3) + (w >> 3) : w;
+    // traversal position -> element: bricks of 2^lx x 2^ly x 2^lz elements,
+    // x fastest inside a brick and across bricks (all edges 1: e = p)
+    const unsigned lx = g.tile[0], ly = g.tile[1], lz = g.tile[2];
+    const unsigned lxy = lx + ly;
+    const unsigned q = p & ((1u << (lxy + lz)) - 1u), tp = p >> (lxy + lz);
+    const unsigned ntx = (unsigned)g.ne[0] >> lx, nty = (unsigned)g.ne[1] >> ly;
+    const unsigned tx = tp % ntx, t2 = tp / ntx, ty = t2 % nty, tz = t2 / nty;
+    const unsigned x = (tx << lx) | (q & ((1u << lx) - 1u));
+    const unsigned y = (ty << ly) | ((q >> lx) & ((1u << ly) - 1u));
+    const unsigned z = (tz << lz) | (q >> lxy);
+    return x + (unsigned)g.ne[0] * (y + (unsigned)g.ne[1] * z);
 }
 
 __device__ __forceinline__ void elem_ijk(unsigned e, const KGeom& g, int* c) {
@@ -643,7 +688,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
         double U[V];
 #pragma unroll
         for (int s = 0; s < V; ++s) U[s] = u[s];
-        const double r1 = 1. / U[0];
+        const double r1 = frcp(U[0]);
         const double p = pressure_r(G.gamma, U, r1);
         double* o = Qd + q * 9;
         o[0] = U[0];
@@ -886,8 +931,9 @@ struct StageLoads {
 };
 
 template <int P, int FLUX>
-__global__ __launch_bounds__(TEAM) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp, KGeom G) {
-    constexpr int TS = TEAM;
+__global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
+                                                                KGeom G) {
+    constexpr int TS = stage_team<P>();
     const KTables& T = *Tp;
     using Dm = Dims<P>;
     using L = StageLDS<P>;
@@ -1195,7 +1241,7 @@ __global__ __launch_bounds__(WAVE) void wavespeed_kernel(const double* U, double
 // ---------------------------------------------------------------------------
 template <int P>
 hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& g, hipStream_t st) {
-    dim3 grid(g.E), block(TEAM);
+    dim3 grid(g.E), block(stage_team<P>());
     switch (flux) {
     case 0: hipLaunchKernelGGL((stage_kernel<P, 0>), grid, block, 0, st, a, t, g); break;
     case 1: hipLaunchKernelGGL((stage_kernel<P, 1>), grid, block, 0, st, a, t, g); break;

@@ -36,6 +36,7 @@ struct KGeom {
     double jdet, jinv;              // |J| = hx hy hz / 8 and its inverse
     double gamma, R;
     int xcd_remap;                  // 1: map workgroups to elements XCD-contiguously
+    int tile[3];                    // log2 brick edges of the stage-kernel traversal
 };
 
 // Initial condition / exact solution selector and data.

@@ -2,6 +2,8 @@
 #include "dg_tables.hpp"
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -184,6 +186,24 @@ KGeom make_geometry(const CaseConfig& c) {
     g.gamma = c.gamma;
     g.R = c.gas_constant;
     g.xcd_remap = (E % 8 == 0 && E >= 8 * 64) ? 1 : 0;
+    // Stage-kernel traversal in bricks of up to 8^3 elements (edge: the
+    // largest power of two <= 8 dividing the element count), so that the
+    // neighbours of the elements in flight on an XCD are still in its L2.
+    // DG_ELEMENT_TILE="lx,ly,lz" (log2 edges) overrides, for sweeps.
+    for (int d = 0; d < 3; ++d) {
+        int l = 0;
+        while (l < 3 && c.nelem[d] % (2 << l) == 0) ++l;
+        g.tile[d] = l;
+    }
+    if (const char* env = std::getenv("DG_ELEMENT_TILE")) {
+        int t[3];
+        if (std::sscanf(env, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3) {
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) ok = ok && t[d] >= 0 && t[d] < 16 && c.nelem[d] % (1 << t[d]) == 0;
+            if (ok)
+                for (int d = 0; d < 3; ++d) g.tile[d] = t[d];
+        }
+    }
     return g;
 }
 
