@@ -712,6 +712,7 @@ struct orc_solver {
     /* face helpers */
     int nqf;
     double *fq_wts, *fq_pts;  /* [nqf], [6][nqf][3] */
+    double* fq_lp;            /* [nqf][2] face parameters (s,t) of the points */
     double* f_bval;           /* [6*nqf][nb] */
     int *qidxL, *qidxR;       /* [F][nqf] */
     double* normals;          /* [F][nqf][3] */
@@ -752,6 +753,20 @@ static void element_jacobian(const double* X, int nq, const double* ggrad,
     }
 }
 
+void orc_box_node(const orc_params* prm, long i, long j, long k, double* x) {
+    const int* ne = prm->nelem;
+    const long ijk[3] = {i, j, k};
+    double h[3], sn[3];
+    for (int d = 0; d < 3; ++d) {
+        h[d] = (prm->hi[d] - prm->lo[d]) / ne[d];
+        sn[d] = sin(2.0 * M_PI * (double)(ijk[d] % ne[d]) / (double)ne[d]);
+    }
+    for (int d = 0; d < 3; ++d) {
+        x[d] = prm->lo[d] + ijk[d] * h[d];
+        if (prm->warp != 0.0) x[d] += prm->warp * h[d] * (sn[(d + 1) % 3] * sn[(d + 2) % 3]);
+    }
+}
+
 /* structured periodic box in the reference mesh conventions
  * (src/mesh/mesh.h:110-156,280-284; faces per shape.h:80-86). */
 static void build_mesh(orc_solver* s) {
@@ -761,13 +776,12 @@ static void build_mesh(orc_solver* s) {
     s->node_coords = xcalloc((size_t)s->nnodes * 3, sizeof(double));
     double h[3];
     for (int d = 0; d < 3; ++d) h[d] = (s->prm.hi[d] - s->prm.lo[d]) / ne[d];
+    (void)h;
     for (long k = 0; k < nz1; ++k)
         for (long j = 0; j < ny1; ++j)
             for (long i = 0; i < nx1; ++i) {
                 long id = i + nx1 * (j + ny1 * k);
-                s->node_coords[id * 3 + 0] = s->prm.lo[0] + i * h[0];
-                s->node_coords[id * 3 + 1] = s->prm.lo[1] + j * h[1];
-                s->node_coords[id * 3 + 2] = s->prm.lo[2] + k * h[2];
+                orc_box_node(&s->prm, i, j, k, s->node_coords + id * 3);
             }
     s->e2n = xcalloc((size_t)s->E * 8, sizeof(long));
     for (long iz = 0; iz < ne[2]; ++iz)
@@ -872,6 +886,7 @@ static void build_normals(orc_solver* s) {
     for (int q = 0; q < nqf; ++q) {
         double vx[2], vy[2], gx[2], gy[2];
         double a = s->fq_pts[q * 3 + 0], b = s->fq_pts[q * 3 + 1];
+        if (s->prm.exact_normals) { a = s->fq_lp[q * 2 + 0]; b = s->fq_lp[q * 2 + 1]; }
         lagrange_val_1d(a, xn, 2, vx); lagrange_val_1d(b, xn, 2, vy);
         lagrange_grad_1d(a, xn, 2, gx); lagrange_grad_1d(b, xn, 2, gy);
         for (int j = 0; j < 2; ++j)
@@ -983,7 +998,7 @@ static void build_face_helpers(orc_solver* s) {
     for (int f = 0; f < 6; ++f) orc_points_on_face(f, 0, nqf, lp, s->fq_pts + (size_t)f * nqf * 3);
     s->f_bval = xcalloc((size_t)6 * nqf * nb, sizeof(double));
     orc_basis_3d(s->prm.basis, s->p, 6 * nqf, s->fq_pts, s->f_bval, NULL);
-    free(lp);
+    s->fq_lp = lp;
 }
 
 orc_solver* orc_create(const orc_params* prm) {
@@ -1023,7 +1038,7 @@ void orc_destroy(orc_solver* s) {
     if (!s) return;
     void* ptrs[] = {s->node_coords, s->e2n, s->ifaces, s->vq_pts, s->vq_wts, s->v_bval,
                     s->v_bgrad, s->djac, s->ijac, s->vol, s->imm, s->fq_wts, s->fq_pts,
-                    s->f_bval, s->qidxL, s->qidxR, s->normals, s->Uc, s->Uq, s->vgUq,
+                    s->f_bval, s->qidxL, s->qidxR, s->normals, s->fq_lp, s->Uc, s->Uq, s->vgUq,
                     s->Uqf, s->UqL, s->UqR, s->Fq, s->Fq_elem, s->res, s->dU1, s->dU2,
                     s->dU3, s->dU4, s->Uhold};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);

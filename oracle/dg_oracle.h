@@ -45,6 +45,13 @@ typedef struct {
     int ic;               /* enum orc_ic                                */
     int n_ic_data;
     double ic_data[10];
+    double warp;          /* 0: affine box; else periodic node displacement
+                             (orc_box_node, a trilinear non-affine mesh) */
+    int exact_normals;    /* 0: normals where the reference evaluates them
+                             (basis.cpp:332, the face-0 points' first two
+                             coordinates, i.e. each face's (s,t) swapped);
+                             1: at the flux points themselves (new).  Same
+                             on affine faces. */
 } orc_params;
 
 typedef struct orc_solver orc_solver;
@@ -72,6 +79,15 @@ void   orc_flux_numerical(int flux, double gamma, const double* UL,
 void   orc_ic_eval(const orc_params* prm, const double* x, double t, double* U);
 
 /* ---- solver ---- */
+/* Node (i, j, k) of the periodic box, i in [0, nelem[0]] etc.:
+ *   x = lo_x + i h_x + warp h_x sin(2 pi eta) sin(2 pi zeta)
+ *   y = lo_y + j h_y + warp h_y sin(2 pi zeta) sin(2 pi xi)
+ *   z = lo_z + k h_z + warp h_z sin(2 pi xi) sin(2 pi eta)
+ * with xi = (i mod nx)/nx, eta = (j mod ny)/ny, zeta = (k mod nz)/nz.  Each
+ * displacement component depends on the other two node indices only, so
+ * opposite box faces stay translates of each other (periodic pairing). */
+void orc_box_node(const orc_params* prm, long i, long j, long k, double* x);
+
 orc_solver* orc_create(const orc_params* prm);
 void   orc_destroy(orc_solver* s);
 long   orc_num_elems(const orc_solver* s);

@@ -22,7 +22,8 @@ class Params(C.Structure):
     _fields_ = [("order", C.c_int), ("basis", C.c_int), ("flux", C.c_int),
                 ("nelem", C.c_int * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
                 ("gamma", C.c_double), ("R", C.c_double), ("ic", C.c_int),
-                ("n_ic_data", C.c_int), ("ic_data", C.c_double * 10)]
+                ("n_ic_data", C.c_int), ("ic_data", C.c_double * 10), ("warp", C.c_double),
+                ("exact_normals", C.c_int)]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -89,8 +90,10 @@ def lib():
 
 def make_params(order=2, basis="LegendreHex", flux="LaxFriedrichs", nelem=(2, 2, 2),
                 lo=(0., 0., 0.), hi=(1., 1., 1.), gamma=1.4, R=1.0,
-                ic="Uniform", ic_data=(1.0, 0.1, 0.2, 0.3, 2.5)):
+                ic="Uniform", ic_data=(1.0, 0.1, 0.2, 0.3, 2.5), warp=0.0, exact_normals=False):
     p = Params()
+    p.warp = float(warp)
+    p.exact_normals = 1 if exact_normals else 0
     p.order = order
     p.basis = BASIS[basis]
     p.flux = FLUX[flux]
