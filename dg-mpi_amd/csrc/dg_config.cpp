@@ -322,6 +322,15 @@ CaseConfig parse_case(const std::string& toml_text) {
         if (!(c.upper[i] > c.lower[i])) throw InputError("Mesh.upper must exceed Mesh.lower");
     if ((double)c.nelem[0] * c.nelem[1] * c.nelem[2] > 2.0e9)
         throw InputError("Mesh.nelem: too many elements");
+    c.warp = get_num(d, "Mesh", "warp", false, 0.0);
+    if (!(c.warp >= 0.0 && c.warp <= 0.25))
+        throw InputError("Mesh.warp must lie in [0, 0.25] (node displacement in element widths)");
+    const std::string nrm = get_str(d, "Mesh", "normals", false, "reference");
+    if (nrm == "exact") c.exact_normals = true;
+    else if (nrm != "reference") throw InputError("Mesh.normals must be \"reference\" or \"exact\"");
+    if (c.warp > 0.0 && c.order > 4)
+        throw UnsupportedError("Mesh.warp with order > 4: the per-element mass matrix inverse of the "
+                               "non-affine path is formed in LDS and holds at most (4+1)^3 basis functions");
     return c;
 }
 

@@ -47,9 +47,14 @@ struct CaseConfig {
     // [Numerics] (src/numerics/numerics_data.cpp:229-300)
     int order = 0;
     Basis basis = Basis::LagrangeHex;
-    // [Mesh]: structured periodic box
+    // [Mesh]: structured periodic box, type = "PeriodicBox", nelem, lower,
+    // upper; warp > 0 displaces the nodes periodically (trilinear,
+    // non-affine elements); normals = "reference" (the reference's
+    // evaluation points) or "exact" (at the flux points)
     int nelem[3] = {0, 0, 0};
     double lower[3] = {0, 0, 0}, upper[3] = {1, 1, 1};
+    double warp = 0.0;
+    bool exact_normals = false;
     // [Stepper] (src/solver/base.cpp:46-57)
     Stepper stepper = Stepper::FE;
     double timestep = 0.0;     // 0 => derived from cfl

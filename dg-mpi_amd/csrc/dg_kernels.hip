@@ -68,6 +68,7 @@ struct LaneTab {
     double phim[PMAX + 1];   // phi_i(-1)
     double phip[PMAX + 1];   // phi_i(+1)
     double mdiag[PMAX + 1];  // diagonal of the 1D inverse mass matrix
+    double xv[PMAX + 2];     // GL nodes of the (p+2)-point rule
 };
 
 template <int T, int TS = WAVE, class F>
@@ -311,6 +312,212 @@ __device__ __forceinline__ void riemann(double gam, const double* UL, const doub
 }
 
 // ---------------------------------------------------------------------------
+// Non-affine (trilinear) elements
+// ---------------------------------------------------------------------------
+
+// Columns a[c][r] = dx_r/dxi_c of the trilinear map of an element with nodes
+// X[n][3] (n = 4k + 2j + i, src/numerics/basis/tools.cpp:215-247) at the
+// reference point (u0, u1, u2); optionally the mapped point x.
+__device__ __forceinline__ void trilinear(const double* X, double u0, double u1, double u2, double a[3][3],
+                                          double* x = nullptr) {
+    const double l0[2] = {0.5 * (1. - u0), 0.5 * (1. + u0)};
+    const double l1[2] = {0.5 * (1. - u1), 0.5 * (1. + u1)};
+    const double l2[2] = {0.5 * (1. - u2), 0.5 * (1. + u2)};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double c0 = 0., c1 = 0., c2 = 0.;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+                c0 += 0.5 * (X[(4 * o + 2 * q + 1) * 3 + r] - X[(4 * o + 2 * q) * 3 + r]) * (l1[q] * l2[o]);
+                c1 += 0.5 * (X[(4 * o + 2 + q) * 3 + r] - X[(4 * o + q) * 3 + r]) * (l0[q] * l2[o]);
+                c2 += 0.5 * (X[(4 + 2 * o + q) * 3 + r] - X[(2 * o + q) * 3 + r]) * (l0[q] * l1[o]);
+            }
+        a[0][r] = c0;
+        a[1][r] = c1;
+        a[2][r] = c2;
+        if (x) {
+            double v = 0.;
+#pragma unroll
+            for (int n = 0; n < 8; ++n) v += X[n * 3 + r] * (l0[n & 1] * l1[(n >> 1) & 1] * l2[n >> 2]);
+            x[r] = v;
+        }
+    }
+}
+
+__device__ __forceinline__ void cross3(const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+// Area-weighted normal of the + face in direction d of an element at the
+// face's tangential reference coordinates (u1, u2) (t1 < t2 the other two
+// directions): the cofactor row d, a_{d+1} x a_{d+2}, on xi_d = +1 -- the
+// reference's dx/ds x dx/dt of the left element's face
+// (src/numerics/basis/shape.cpp:482-532), outward from it.
+__device__ __forceinline__ void face_normal(const double* X, int d, double u1, double u2, double* N) {
+    double a[3][3];
+    if (d == 0) trilinear(X, 1., u1, u2, a);
+    else if (d == 1) trilinear(X, u1, 1., u2, a);
+    else trilinear(X, u1, u2, 1., a);
+    cross3(a[(d + 1) % 3], a[(d + 2) % 3], N);
+}
+
+// Euler flux dotted with the unit normal n, from 1/rho and p.
+__device__ __forceinline__ void flux_dot(const double* U, double r1, double P, const double* n, double* F) {
+    const double un = (U[1] * n[0] + U[2] * n[1] + U[3] * n[2]) * r1;
+    F[0] = U[0] * un;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) F[1 + i] = U[1 + i] * un + P * n[i];
+    F[4] = (U[4] + P) * un;
+}
+
+// The three numerical fluxes for a general area-weighted normal N, |N|
+// F(n_hat): the formulas of the axis-aligned versions above
+// (src/physics/base/functions.cpp:33-89, src/physics/euler/functions.cpp:
+// 298-399, the Roe-Pike flux of riemann_roe).
+__device__ __forceinline__ void riemann_llf_n(double gam, const double* UL, const double* UR, const double* N,
+                                              double* F) {
+    const double nmag = fsqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]), in = frcp(nmag);
+    const double n[3] = {N[0] * in, N[1] * in, N[2] * in};
+    const double r1L = frcp(UL[0]), r1R = frcp(UR[0]);
+    const double PL = pressure_r(gam, UL, r1L), PR = pressure_r(gam, UR, r1R);
+    double FL[V], FR[V];
+    flux_dot(UL, r1L, PL, n, FL);
+    flux_dot(UR, r1R, PR, n, FR);
+    double a = fsqrt(UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) * r1L + fsqrt(gam * PL * r1L);
+    const double aR = fsqrt(UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]) * r1R + fsqrt(gam * PR * r1R);
+    if (aR > a) a = aR;
+#pragma unroll
+    for (int s = 0; s < V; ++s) F[s] = nmag * (0.5 * (FL[s] + FR[s]) - 0.5 * a * (UR[s] - UL[s]));
+}
+
+__device__ __forceinline__ void riemann_hllc_n(double gam, const double* UL, const double* UR, const double* N,
+                                               double* F) {
+    const double djac = fsqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]), in = frcp(djac);
+    const double n[3] = {N[0] * in, N[1] * in, N[2] * in};
+    const double rhoL = UL[0], rhoL1 = frcp(rhoL);
+    const double unL = (UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2]) * rhoL1;
+    const double PL = pressure_r(gam, UL, rhoL1);
+    const double aL = fsqrt(gam * PL * rhoL1);
+    const double rhoR = UR[0], rhoR1 = frcp(rhoR);
+    const double unR = (UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2]) * rhoR1;
+    const double PR = pressure_r(gam, UR, rhoR1);
+    const double aR = fsqrt(gam * PR * rhoR1);
+    const double rAvg = 0.5 * (rhoL + rhoR);
+    const double aAvg = 0.5 * (aL + aR);
+    const double pStar = fmax(0., 0.5 * (PL + PR - (unR - unL) * rAvg * aAvg));
+    const double gq = (gam + 1.) / (2. * gam);
+    double qL = 1.0;
+    const double xL = pStar * frcp(PL);
+    if (xL > 1.) qL = fsqrt(1. + gq * (xL - 1.));
+    double qR = 1.0;
+    const double xR = pStar * frcp(PR);
+    if (xR > 1.) qR = fsqrt(1. + gq * (xR - 1.));
+    const double sL = unL - aL * qL;
+    const double sR = unR + aR * qR;
+    const double sStar = (PR - PL + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR)) *
+                         frcp(rhoL * (sL - unL) - rhoR * (sR - unR));
+    if (sL >= 0.) {
+        F[0] = djac * rhoL * unL;
+#pragma unroll
+        for (int i = 0; i < 3; i++) F[1 + i] = djac * (UL[1 + i] * unL + PL * n[i]);
+        F[4] = djac * ((UL[4] + PL) * unL);
+    } else if (sR <= 0.) {
+        F[0] = djac * rhoR * unR;
+#pragma unroll
+        for (int i = 0; i < 3; i++) F[1 + i] = djac * (UR[1 + i] * unR + PR * n[i]);
+        F[4] = djac * (UR[4] + PR) * unR;
+    } else if ((sL < 0.) && (sStar >= 0.)) {
+        const double c = (sL - unL) * frcp(sL - sStar);
+        const double f1 = sStar - unL;
+        const double f2 = sStar + PL * frcp(rhoL * (sL - unL));
+        F[0] = djac * (rhoL * unL + rhoL * sL * (c - 1.));
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            F[1 + i] = djac * (UL[1 + i] * unL + PL * n[i] + sL * (UL[1 + i] * (c - 1.) + rhoL * c * f1 * n[i]));
+        F[4] = djac * ((UL[4] + PL) * unL + sL * (UL[4] * (c - 1.) + rhoL * c * f1 * f2));
+    } else if ((sStar < 0.) && (sR > 0.)) {
+        const double c = (sR - unR) * frcp(sR - sStar);
+        const double f1 = sStar - unR;
+        const double f2 = sStar + PR * frcp(rhoR * (sR - unR));
+        F[0] = djac * (rhoR * unR + rhoR * sR * (c - 1.));
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            F[1 + i] = djac * (UR[1 + i] * unR + PR * n[i] + sR * (UR[1 + i] * (c - 1.) + rhoR * c * f1 * n[i]));
+        F[4] = djac * ((UR[4] + PR) * unR + sR * (UR[4] * (c - 1.) + rhoR * c * f1 * f2));
+    } else {
+#pragma unroll
+        for (int s = 0; s < V; ++s) F[s] = __builtin_nan("");
+    }
+}
+
+__device__ __forceinline__ void riemann_roe_n(double gam, const double* UL, const double* UR, const double* N,
+                                              double* F) {
+    const double nmag = fsqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]), in = frcp(nmag);
+    const double n[3] = {N[0] * in, N[1] * in, N[2] * in};
+    const double rL = UL[0], rR = UR[0];
+    const double r1L = frcp(rL), r1R = frcp(rR);
+    double uL[3], uR[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        uL[d] = UL[1 + d] * r1L;
+        uR[d] = UR[1 + d] * r1R;
+    }
+    const double gm1 = gam - 1.;
+    const double pL = gm1 * (UL[4] - 0.5 * (UL[1] * uL[0] + UL[2] * uL[1] + UL[3] * uL[2]));
+    const double pR = gm1 * (UR[4] - 0.5 * (UR[1] * uR[0] + UR[2] * uR[1] + UR[3] * uR[2]));
+    const double HL = (UL[4] + pL) * r1L, HR = (UR[4] + pR) * r1R;
+    const double sL = fsqrt(rL), sR = fsqrt(rR), isum = frcp(sL + sR);
+    double ut[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) ut[d] = (sL * uL[d] + sR * uR[d]) * isum;
+    const double Ht = (sL * HL + sR * HR) * isum;
+    const double q2 = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
+    const double c2 = gm1 * (Ht - 0.5 * q2);
+    const double ct = fsqrt(c2);
+    const double ic2 = frcp(c2);
+    const double rt = sL * sR;
+    const double unt = ut[0] * n[0] + ut[1] * n[1] + ut[2] * n[2];
+    const double unL = uL[0] * n[0] + uL[1] * n[1] + uL[2] * n[2];
+    const double unR = uR[0] * n[0] + uR[1] * n[1] + uR[2] * n[2];
+    const double drho = rR - rL, dp = pR - pL, dun = unR - unL;
+    const double a1 = (dp - rt * ct * dun) * (0.5 * ic2);
+    const double a5 = (dp + rt * ct * dun) * (0.5 * ic2);
+    const double a2 = drho - dp * ic2;
+    const double cL = fsqrt(gam * pL * r1L), cR = fsqrt(gam * pR * r1R);
+    const double l1 = entropy_fix(unt - ct, unL - cL, unR - cR);
+    const double l2 = fabs(unt);
+    const double l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
+    const double w1 = l1 * a1, w5 = l5 * a5, w2 = l2 * a2, ws = l2 * rt;
+    double Dv[V];
+    Dv[0] = w1 + w5 + w2;
+    double udu = 0.;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double dut = (uR[d] - uL[d]) - dun * n[d];
+        Dv[1 + d] = w1 * (ut[d] - ct * n[d]) + w5 * (ut[d] + ct * n[d]) + w2 * ut[d] + ws * dut;
+        udu += ut[d] * dut;
+    }
+    Dv[4] = w1 * (Ht - unt * ct) + w5 * (Ht + unt * ct) + w2 * 0.5 * q2 + ws * udu;
+    double FL[V], FR[V];
+    flux_dot(UL, r1L, pL, n, FL);
+    flux_dot(UR, r1R, pR, n, FR);
+#pragma unroll
+    for (int s = 0; s < V; ++s) F[s] = nmag * (0.5 * (FL[s] + FR[s]) - 0.5 * Dv[s]);
+}
+
+template <int FLUX>
+__device__ __forceinline__ void riemann_n(double gam, const double* UL, const double* UR, const double* N,
+                                          double* F) {
+    if constexpr (FLUX == 0) riemann_llf_n(gam, UL, UR, N, F);
+    else if constexpr (FLUX == 1) riemann_hllc_n(gam, UL, UR, N, F);
+    else riemann_roe_n(gam, UL, UR, N, F);
+}
+
+// ---------------------------------------------------------------------------
 // Initial conditions / exact solutions (same formulas as the oracle's
 // orc_ic_eval; reference: src/physics/base/base.cpp:150-194,
 // src/physics/euler/functions.cpp:94-191)
@@ -446,7 +653,7 @@ __device__ __forceinline__ constexpr int bidx(int r, int t1, int t2) {
 // ---------------------------------------------------------------------------
 // LDS layout of the stage kernel
 // ---------------------------------------------------------------------------
-template <int P>
+template <int P, bool MAPPED = false>
 struct StageLDS {
     using Dm = Dims<P>;
     static constexpr int N = Dm::N, M = Dm::M;
@@ -464,7 +671,8 @@ struct StageLDS {
     static constexpr int X1 = N * N * M * V;
     static constexpr int X2 = N * M * M * V;
     static constexpr int UQ = Dm::MMM * V;
-    static constexpr int QD = Dm::MMM * 9;
+    static constexpr int QW = MAPPED ? 17 : 9;      // point data per volume point
+    static constexpr int QD = Dm::MMM * QW;
     static constexpr int C = 3 * N * M * M * V;
     static constexpr int BV = 2 * N * N * M * V;
     // bank A holds U at offset 0 during the face phase
@@ -493,9 +701,10 @@ __device__ __forceinline__ int bidx_d(int d, int r, int t1, int t2) {
 // arithmetic of the neighbour's own slab, so both sides of a face see the
 // same trace and the same flux.
 // ---------------------------------------------------------------------------
-template <int P, int FLUX, int TS>
+template <int P, int FLUX, int TS, bool MAPPED = false>
 __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, const KGeom& G, int lane,
-                                           const double* sU, double* sR, double* bankA, double* bankB) {
+                                           const double* sU, double* sR, double* bankA, double* bankB,
+                                           const double* XL = nullptr) {
     using Dm = Dims<P>;
     using L = StageLDS<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, NNN = Dm::NNN;
@@ -565,9 +774,26 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
             UL[s] = side == 0 ? nbv[s] : own[s];
             UR[s] = side == 0 ? own[s] : nbv[s];
         }
-        if (d == 0) riemann<FLUX, 0>(G.gamma, UL, UR, G.farea[0], F);
-        else if (d == 1) riemann<FLUX, 1>(G.gamma, UL, UR, G.farea[1], F);
-        else riemann<FLUX, 2>(G.gamma, UL, UR, G.farea[2], F);
+        if constexpr (MAPPED) {
+            // normal of the face's left element (own + face, or the + face
+            // of the neighbour across the - face), where the reference
+            // evaluates it: at each face's (s, t) swapped (basis.cpp:332;
+            // s, t are (xi_y, xi_z), (-xi_x, xi_z), (xi_x, xi_y) for the +x,
+            // +y, +z faces, shape.cpp:296-323), or at the flux point
+            const double* XLe = XL + (side ? 0 : (1 + d) * 24);
+            double u1 = LT.xv[a1], u2 = LT.xv[a2];
+            if (!G.exact_normals) {
+                u1 = d == 1 ? -LT.xv[a2] : LT.xv[a2];
+                u2 = d == 1 ? -LT.xv[a1] : LT.xv[a1];
+            }
+            double Nv[3];
+            face_normal(XLe, d, u1, u2, Nv);
+            riemann_n<FLUX>(G.gamma, UL, UR, Nv, F);
+        } else {
+            if (d == 0) riemann<FLUX, 0>(G.gamma, UL, UR, G.farea[0], F);
+            else if (d == 1) riemann<FLUX, 1>(G.gamma, UL, UR, G.farea[1], F);
+            else riemann<FLUX, 2>(G.gamma, UL, UR, G.farea[2], F);
+        }
         const double wq = LT.w[a1] * LT.w[a2];
         double* o = fh + (((d * 2 + side) * M + a2) * M + a1) * V;
 #pragma unroll
@@ -624,9 +850,10 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
 // ---------------------------------------------------------------------------
 // Volume phase: sum-factorised interpolation, point fluxes, weak gradient.
 // ---------------------------------------------------------------------------
-template <int P, int TS>
+template <int P, int TS, bool MAPPED = false>
 __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT, const KGeom& G, int lane,
-                                             double* sR, double* bankA, double* bankB) {
+                                             double* sR, double* bankA, double* bankB,
+                                             const double* XL = nullptr) {
     using Dm = Dims<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, MMM = Dm::MMM;
     const double* sU = bankA;
@@ -682,7 +909,39 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
         }
     });
     sync();
-    // point data: (rho, m_x, m_y, m_z, E + p, u, v, w, p)
+    // point data: (rho, m_x, m_y, m_z, E + p, u, v, w, p); on mapped
+    // elements (rho, m, E + p, C_d . u (d = 0..2), p C_d (d = 0..2)) with the
+    // cofactor rows C_d = a_{d+1} x a_{d+2} = |J| dxi_d/dx at the point, so
+    // that the contravariant flux G_d = sum_j F_j (J^-1)_dj |J| of
+    // flux_functors_impl.h:95-102 is W C_d.u + p C_d
+    if constexpr (MAPPED) {
+        team_for<MMM, TS>(lane, [&](int q) {
+            const double* u = Uq + q * V;
+            double U[V];
+#pragma unroll
+            for (int s = 0; s < V; ++s) U[s] = u[s];
+            const double r1 = frcp(U[0]);
+            const double p = pressure_r(G.gamma, U, r1);
+            const int a = q % M, b = (q / M) % M, c = q / MM;
+            double J[3][3];
+            trilinear(XL, LT.xv[a], LT.xv[b], LT.xv[c], J);
+            double* o = Qd + q * 17;
+            o[0] = U[0];
+            o[1] = U[1];
+            o[2] = U[2];
+            o[3] = U[3];
+            o[4] = U[4] + p;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double Cd[3];
+                cross3(J[(d + 1) % 3], J[(d + 2) % 3], Cd);
+                o[5 + d] = (Cd[0] * U[1] + Cd[1] * U[2] + Cd[2] * U[3]) * r1;
+                o[8 + 3 * d] = p * Cd[0];
+                o[9 + 3 * d] = p * Cd[1];
+                o[10 + 3 * d] = p * Cd[2];
+            }
+        });
+    } else
     team_for<MMM, TS>(lane, [&](int q) {
         const double* u = Uq + q * V;
         double U[V];
@@ -712,12 +971,24 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
         for (int k = 0; k < N; ++k) xx[k] = yy[k] = zz[k] = 0.;
 #pragma unroll
         for (int c = 0; c < M; ++c) {
-            const double* q = Qd + (c * MM + ba) * 9;
-            const double Ws = q[s], p = q[8];
             const double wq = wab * T.w[c];
-            const double gx = (Ws * q[5] + (s == 1 ? p : 0.)) * (G.gvol[0] * wq);
-            const double gy = (Ws * q[6] + (s == 2 ? p : 0.)) * (G.gvol[1] * wq);
-            const double gz = (Ws * q[7] + (s == 3 ? p : 0.)) * (G.gvol[2] * wq);
+            double gx, gy, gz;
+            if constexpr (MAPPED) {
+                const double* q = Qd + (c * MM + ba) * 17;
+                const double Ws = q[s];
+                const double px = (s >= 1 && s <= 3) ? q[8 + (s - 1)] : 0.;
+                const double py = (s >= 1 && s <= 3) ? q[11 + (s - 1)] : 0.;
+                const double pz = (s >= 1 && s <= 3) ? q[14 + (s - 1)] : 0.;
+                gx = (Ws * q[5] + px) * wq;
+                gy = (Ws * q[6] + py) * wq;
+                gz = (Ws * q[7] + pz) * wq;
+            } else {
+                const double* q = Qd + (c * MM + ba) * 9;
+                const double Ws = q[s], p = q[8];
+                gx = (Ws * q[5] + (s == 1 ? p : 0.)) * (G.gvol[0] * wq);
+                gy = (Ws * q[6] + (s == 2 ? p : 0.)) * (G.gvol[1] * wq);
+                gz = (Ws * q[7] + (s == 3 ? p : 0.)) * (G.gvol[2] * wq);
+            }
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 xx[k] = fma(T.B[c][k], gx, xx[k]);
@@ -930,17 +1201,63 @@ struct StageLoads {
     }
 };
 
-template <int P, int FLUX>
+// Dense M^-1 of a mapped element applied in place to an LDS coefficient
+// block r[NB][V] (SolverTools::mult_inv_mass_matrix, src/solver/tools.cpp:
+// 124-139): r <- imm r with the element's symmetric inverse mass matrix in
+// global memory (row b' contiguous, imm[b'][b] = imm[b][b'], so threads of
+// consecutive b read consecutive addresses).  The sum over b' is split in
+// up to four chunks over the threads; part holds chunks * NB * V doubles.
+template <int P, int TS>
+constexpr int dense_part_size() {
+    constexpr int NB = Dims<P>::NNN;
+    constexpr int CH0 = TS / NB;
+    constexpr int CH = CH0 < 1 ? 1 : (CH0 > 4 ? 4 : CH0);
+    return CH * NB * V;
+}
+
+template <int P, int TS>
+__device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int lane, double* r, double* part) {
+    constexpr int NB = Dims<P>::NNN;
+    constexpr int CH0 = TS / NB;
+    constexpr int CH = CH0 < 1 ? 1 : (CH0 > 4 ? 4 : CH0);
+    constexpr int LEN = (NB + CH - 1) / CH;
+    team_for<NB * CH, TS>(lane, [&](int t) {
+        const int b = t % NB, ch = t / NB;
+        const int b0 = ch * LEN, b1 = b0 + LEN < NB ? b0 + LEN : NB;
+        double acc[V] = {0., 0., 0., 0., 0.};
+        for (int bp = b0; bp < b1; ++bp) {
+            const double m = imm[(size_t)bp * NB + b];
+#pragma unroll
+            for (int s = 0; s < V; ++s) acc[s] = fma(m, r[bp * V + s], acc[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < V; ++s) part[(ch * NB + b) * V + s] = acc[s];
+    });
+    sync();
+    team_for<NB * V, TS>(lane, [&](int t) {
+        const int b = t / V, s = t % V;
+        double v = 0.;
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) v += part[(ch * NB + b) * V + s];
+        r[t] = v;
+    });
+    sync();
+}
+
+template <int P, int FLUX, bool MAPPED>
 __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
     constexpr int TS = stage_team<P>();
     const KTables& T = *Tp;
     using Dm = Dims<P>;
-    using L = StageLDS<P>;
+    using L = StageLDS<P, MAPPED>;
     using PF = StageLoads<P, TS>;
     constexpr int NB = Dm::NNN * V;
     __shared__ double lds[L::TOTAL];
     __shared__ LaneTab LT;
+    // mapped: node coordinates of the element and of its -x, -y, -z
+    // neighbours (the left elements of its - faces)
+    __shared__ double XL[MAPPED ? 96 : 1];
     double* sR = lds;
     double* bankA = lds + L::R;
     double* bankB = bankA + L::A;
@@ -951,11 +1268,27 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     // lane tables first (their loads are the first to return), then every
     // global load of the element at once: its block, the neighbour lines
     // and the RK registers
-    if (lane < PMAX + 2) LT.w[lane] = T.w[lane];
+    if (lane < PMAX + 2) {
+        LT.w[lane] = T.w[lane];
+        LT.xv[lane] = T.xv[lane];
+    }
     if (lane < PMAX + 1) {
         LT.phim[lane] = T.phim[lane];
         LT.phip[lane] = T.phip[lane];
         LT.mdiag[lane] = T.minv[lane][lane];
+    }
+    if constexpr (MAPPED) {
+        if (lane < 96) {
+            const int which = lane / 24;
+            unsigned el = e;
+            if (which > 0) {
+                int ec[3];
+                elem_ijk(e, G, ec);
+                ec[which - 1] -= 1;
+                el = elem_at(ec[0], ec[1], ec[2], G);
+            }
+            XL[lane] = G.X[(size_t)el * 24 + lane % 24];
+        }
     }
     PF pf;
     pf.load_block(A.Ust, e, lane);
@@ -973,8 +1306,8 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     }
     pf.store_neighbour_slabs(LT, bankB, lane);
     sync();
-    face_phase<P, FLUX, TS>(T, LT, G, lane, bankA, sR, bankA, bankB);
-    volume_phase<P, TS>(T, LT, G, lane, sR, bankA, bankB);
+    face_phase<P, FLUX, TS, MAPPED>(T, LT, G, lane, bankA, sR, bankA, bankB, XL);
+    volume_phase<P, TS, MAPPED>(T, LT, G, lane, sR, bankA, bankB, XL);
 
     // Epilogue with one store site per array; the value and the target are
     // picked by selects on the (uniform) mode.  Written as one branch per
@@ -982,8 +1315,12 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     // register was left unset on the forward-Euler path.
     const size_t base = (size_t)e * NB;
     const int mode = A.mode;
-    if (rk && !T.minv_diag) minv_lines<P, TS>(T, lane, sR);
-    const double scale = A.dt * G.jinv;
+    if constexpr (MAPPED) {
+        if (rk) dense_minv<P, TS>(G.imm + (size_t)e * Dm::NNN * Dm::NNN, lane, sR, bankA);
+    } else {
+        if (rk && !T.minv_diag) minv_lines<P, TS>(T, lane, sR);
+    }
+    const double scale = MAPPED ? A.dt : A.dt * G.jinv;
     double* const out = rk ? A.Uout : A.R;
     const bool wacc = mode == MODE_RK_FIRST || mode == MODE_RK_MID;
 #pragma unroll
@@ -991,7 +1328,7 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
         const int t = r * TS + lane;
         if (t < NB) {
             const double rv = sR[t];
-            const double sc = T.minv_diag ? scale * lane_diag_scale<P>(LT, t / V) : scale;
+            const double sc = (!MAPPED && T.minv_diag) ? scale * lane_diag_scale<P>(LT, t / V) : scale;
             const double dU = sc * rv;
             const double u0 = pf.u0[r], ac = pf.ac[r];
             const double n_rk = u0 + A.alpha * dU;                 // U0 + alpha dU
@@ -1011,10 +1348,17 @@ __global__ __launch_bounds__(WAVE) void inv_mass_kernel(double dt, const double*
     const KTables& T = *Tp;
     constexpr int NB = Dims<P>::NNN * V;
     __shared__ double sR[NB];
+    __shared__ double part[dense_part_size<P, WAVE>()];
     const int lane = threadIdx.x;
     const size_t base = (size_t)blockIdx.x * NB;
     team_for<NB>(lane, [&](int t) { sR[t] = R[base + t]; });
     sync();
+    if (G.mapped) {
+        constexpr int NN3 = Dims<P>::NNN;
+        dense_minv<P, WAVE>(G.imm + (size_t)blockIdx.x * NN3 * NN3, lane, sR, part);
+        team_for<NB>(lane, [&](int t) { dU[base + t] = dt * sR[t]; });
+        return;
+    }
     if (!T.minv_diag) minv_lines<P, WAVE>(T, lane, sR);
     const double scale = dt * G.jinv;
     team_for<NB>(lane, [&](int t) {
@@ -1076,20 +1420,33 @@ __global__ __launch_bounds__(WAVE) void init_state_kernel(double time, double* U
     __shared__ double rhs[NB];
     __shared__ double fs[Q * Q * V];
     __shared__ double t1[Q * N * V];
+    __shared__ double Xe[24];
+    __shared__ double part[dense_part_size<P, WAVE>()];
     const int lane = threadIdx.x;
     const unsigned e = blockIdx.x;
     int ec[3];
     elem_ijk(e, G, ec);
+    if (G.mapped && lane < 24) Xe[lane] = G.X[(size_t)e * 24 + lane];
     team_for<NB>(lane, [&](int t) { rhs[t] = 0.; });
+    sync();
     for (int c = 0; c < Q; ++c) {
         // f w |J| at the slice points (src/solver/tools.cpp:348-352)
         team_for<Q * Q>(lane, [&](int t) {
             const int a = t % Q, b = t / Q;
-            const double x[3] = {phys(G, 0, ec[0], T.xq[a]), phys(G, 1, ec[1], T.xq[b]),
-                                 phys(G, 2, ec[2], T.xq[c])};
+            double x[3], jd = G.jdet;
+            if (G.mapped) {
+                double J[3][3], C0[3];
+                trilinear(Xe, T.xq[a], T.xq[b], T.xq[c], J, x);
+                cross3(J[1], J[2], C0);
+                jd = J[0][0] * C0[0] + J[0][1] * C0[1] + J[0][2] * C0[2];
+            } else {
+                x[0] = phys(G, 0, ec[0], T.xq[a]);
+                x[1] = phys(G, 1, ec[1], T.xq[b]);
+                x[2] = phys(G, 2, ec[2], T.xq[c]);
+            }
             double f[V];
             ic_eval(ic, G, x, time, f);
-            const double wd = T.wq[a] * T.wq[b] * T.wq[c] * G.jdet;
+            const double wd = T.wq[a] * T.wq[b] * T.wq[c] * jd;
 #pragma unroll
             for (int s = 0; s < V; ++s) fs[(b * Q + a) * V + s] = wd * f[s];
         });
@@ -1126,8 +1483,13 @@ __global__ __launch_bounds__(WAVE) void init_state_kernel(double time, double* U
         });
         sync();
     }
-    if (!T.minv_diag) minv_lines<P, WAVE>(T, lane, rhs);
     const size_t base = (size_t)e * NB;
+    if (G.mapped) {
+        dense_minv<P, WAVE>(G.imm + (size_t)e * NNN * NNN, lane, rhs, part);
+        team_for<NB>(lane, [&](int t) { U[base + t] = rhs[t]; });
+        return;
+    }
+    if (!T.minv_diag) minv_lines<P, WAVE>(T, lane, rhs);
     team_for<NB>(lane, [&](int t) {
         const double sc = T.minv_diag ? G.jinv * minv_diag_scale<P>(T, t / V) : G.jinv;
         U[base + t] = sc * rhs[t];
@@ -1147,11 +1509,13 @@ __global__ __launch_bounds__(WAVE) void eval_kernel(int mode, double time, const
     __shared__ double t1[N * N * Q * V];
     __shared__ double t2[N * Q * Q * V];
     __shared__ double red[WAVE * NO];
+    __shared__ double Xe[24];
     const int lane = threadIdx.x;
     const unsigned e = blockIdx.x;
     int ec[3];
     elem_ijk(e, G, ec);
     const size_t base = (size_t)e * NB;
+    if (G.mapped && lane < 24) Xe[lane] = G.X[(size_t)e * 24 + lane];
     team_for<NB>(lane, [&](int t) { su[t] = U[base + t]; });
     sync();
     contract_xy<P, Q>(T.BQ, lane, su, t1, t2);
@@ -1167,10 +1531,19 @@ __global__ __launch_bounds__(WAVE) void eval_kernel(int mode, double time, const
                 for (int k = 0; k < N; ++k) acc = fma(T.BQ[c][k], t2[((k * Q + b) * Q + a) * V + s], acc);
                 u[s] = acc;
             }
-            const double wd = T.wq[a] * T.wq[b] * T.wq[c] * G.jdet;
+            double x[3], jd = G.jdet;
+            if (G.mapped) {
+                double J[3][3], C0[3];
+                trilinear(Xe, T.xq[a], T.xq[b], T.xq[c], J, x);
+                cross3(J[1], J[2], C0);
+                jd = J[0][0] * C0[0] + J[0][1] * C0[1] + J[0][2] * C0[2];
+            } else {
+                x[0] = phys(G, 0, ec[0], T.xq[a]);
+                x[1] = phys(G, 1, ec[1], T.xq[b]);
+                x[2] = phys(G, 2, ec[2], T.xq[c]);
+            }
+            const double wd = T.wq[a] * T.wq[b] * T.wq[c] * jd;
             if (mode == 0) {
-                const double x[3] = {phys(G, 0, ec[0], T.xq[a]), phys(G, 1, ec[1], T.xq[b]),
-                                     phys(G, 2, ec[2], T.xq[c])};
                 double ue[V];
                 ic_eval(ic, G, x, time, ue);
 #pragma unroll
@@ -1178,6 +1551,7 @@ __global__ __launch_bounds__(WAVE) void eval_kernel(int mode, double time, const
                     const double d = u[s] - ue[s];
                     accv[s] += d * d * wd;
                 }
+                accv[5] += wd;   // element volume
             } else {
 #pragma unroll
                 for (int s = 0; s < V; ++s) accv[s] += u[s] * wd;
@@ -1236,16 +1610,109 @@ __global__ __launch_bounds__(WAVE) void wavespeed_kernel(const double* U, double
     }
 }
 
+// Mass matrix of a mapped element on the (2p+2)^3 rule, M = sum_q w_q |J|_q
+// phi(q) phi(q)^T (src/solver/helpers.cpp:53-100,
+// src/numerics/basis/tools.cpp:85-110), sum-factorised, then inverted in
+// place by Gauss-Jordan elimination (M is symmetric positive definite: no
+// pivoting).  One workgroup per element; jmin[e] = smallest |J| at the
+// points.
+template <int P>
+__global__ __launch_bounds__(256) void mass_kernel(double* imm, double* jmin, const KTables* __restrict__ Tp,
+                                                   KGeom G) {
+    constexpr int TS = 256;
+    const KTables& T = *Tp;
+    using Dm = Dims<P>;
+    constexpr int N = Dm::N, NN = Dm::NN, NB = Dm::NNN, Q = Dm::Q1;
+    __shared__ double Xe[24];
+    __shared__ double W[Q * Q * Q];          // [c][b][a]
+    __shared__ double T1[NN * Q * Q];        // [(i1, i2)][c][b]
+    __shared__ double A[NB * NB];
+    __shared__ double col[NB];
+    __shared__ double red[TS];
+    const int lane = threadIdx.x;
+    const unsigned e = blockIdx.x;
+    if (lane < 24) Xe[lane] = G.X[(size_t)e * 24 + lane];
+    sync();
+    double jm = 1e300;
+    team_for<Q * Q * Q, TS>(lane, [&](int t) {
+        const int a = t % Q, b = (t / Q) % Q, c = t / (Q * Q);
+        double J[3][3], C0[3];
+        trilinear(Xe, T.xq[a], T.xq[b], T.xq[c], J);
+        cross3(J[1], J[2], C0);
+        const double jd = J[0][0] * C0[0] + J[0][1] * C0[1] + J[0][2] * C0[2];
+        jm = fmin(jm, jd);
+        W[t] = T.wq[a] * T.wq[b] * T.wq[c] * jd;
+    });
+    red[lane] = jm;
+    sync();
+    if (lane == 0) {
+        double m = red[0];
+        for (int l = 1; l < TS; ++l) m = fmin(m, red[l]);
+        jmin[e] = m;
+    }
+    // T1[(i1, i2)][c][b] = sum_a BQ[a][i1] BQ[a][i2] W[c][b][a]
+    team_for<NN * Q * Q, TS>(lane, [&](int t) {
+        const int cb = t % (Q * Q), ii = t / (Q * Q), i1 = ii / N, i2 = ii % N;
+        double acc = 0.;
+#pragma unroll
+        for (int a = 0; a < Q; ++a) acc = fma(T.BQ[a][i1] * T.BQ[a][i2], W[cb * Q + a], acc);
+        T1[t] = acc;
+    });
+    sync();
+    // M[(k1 j1 i1)][(k2 j2 i2)] = sum_c BQ[c][k1] BQ[c][k2] sum_b BQ[b][j1] BQ[b][j2] T1[(i1, i2)][c][b]
+    team_for<NB * NB, TS>(lane, [&](int t) {
+        const int r = t / NB, q = t % NB;
+        const int i1 = r % N, j1 = (r / N) % N, k1 = r / NN;
+        const int i2 = q % N, j2 = (q / N) % N, k2 = q / NN;
+        const double* t1 = T1 + (i1 * N + i2) * Q * Q;
+        double acc = 0.;
+        for (int c = 0; c < Q; ++c) {
+            double in = 0.;
+#pragma unroll
+            for (int b = 0; b < Q; ++b) in = fma(T.BQ[b][j1] * T.BQ[b][j2], t1[c * Q + b], in);
+            acc = fma(T.BQ[c][k1] * T.BQ[c][k2], in, acc);
+        }
+        A[t] = acc;
+    });
+    sync();
+    for (int k = 0; k < NB; ++k) {
+        team_for<NB, TS>(lane, [&](int i) { col[i] = A[i * NB + k]; });
+        sync();
+        const double ip = 1.0 / col[k];
+        team_for<NB, TS>(lane, [&](int j) { A[k * NB + j] = (j == k ? 1.0 : A[k * NB + j]) * ip; });
+        sync();
+        team_for<NB * NB, TS>(lane, [&](int t) {
+            const int i = t / NB, j = t % NB;
+            if (i != k) A[t] = (j == k ? 0.0 : A[t]) - col[i] * A[k * NB + j];
+        });
+        sync();
+    }
+    team_for<NB * NB, TS>(lane, [&](int t) { imm[(size_t)e * NB * NB + t] = A[t]; });
+}
+
 // ---------------------------------------------------------------------------
 // Dispatch
 // ---------------------------------------------------------------------------
 template <int P>
 hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& g, hipStream_t st) {
     dim3 grid(g.E), block(stage_team<P>());
-    switch (flux) {
-    case 0: hipLaunchKernelGGL((stage_kernel<P, 0>), grid, block, 0, st, a, t, g); break;
-    case 1: hipLaunchKernelGGL((stage_kernel<P, 1>), grid, block, 0, st, a, t, g); break;
-    default: hipLaunchKernelGGL((stage_kernel<P, 2>), grid, block, 0, st, a, t, g); break;
+    if (g.mapped) {
+        // non-affine elements: orders up to 4 (the case parser rejects more)
+        if constexpr (P <= 4) {
+            switch (flux) {
+            case 0: hipLaunchKernelGGL((stage_kernel<P, 0, true>), grid, block, 0, st, a, t, g); break;
+            case 1: hipLaunchKernelGGL((stage_kernel<P, 1, true>), grid, block, 0, st, a, t, g); break;
+            default: hipLaunchKernelGGL((stage_kernel<P, 2, true>), grid, block, 0, st, a, t, g); break;
+            }
+        } else {
+            return hipErrorInvalidValue;
+        }
+    } else {
+        switch (flux) {
+        case 0: hipLaunchKernelGGL((stage_kernel<P, 0, false>), grid, block, 0, st, a, t, g); break;
+        case 1: hipLaunchKernelGGL((stage_kernel<P, 1, false>), grid, block, 0, st, a, t, g); break;
+        default: hipLaunchKernelGGL((stage_kernel<P, 2, false>), grid, block, 0, st, a, t, g); break;
+        }
     }
     return hipGetLastError();
 }
@@ -1289,6 +1756,18 @@ hipError_t launch_eval(int P, int mode, double time, const double* U, double* ou
 hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables* t, const KGeom& g,
                             hipStream_t st) {
     DG_DISPATCH_P(P, hipLaunchKernelGGL((wavespeed_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, U, out, t, g));
+    return hipGetLastError();
+}
+
+hipError_t launch_mass(int P, double* imm, double* jmin, const KTables* t, const KGeom& g, hipStream_t st) {
+    const dim3 grid(g.E), block(256);
+    switch (P) {
+    case 1: hipLaunchKernelGGL((mass_kernel<1>), grid, block, 0, st, imm, jmin, t, g); break;
+    case 2: hipLaunchKernelGGL((mass_kernel<2>), grid, block, 0, st, imm, jmin, t, g); break;
+    case 3: hipLaunchKernelGGL((mass_kernel<3>), grid, block, 0, st, imm, jmin, t, g); break;
+    case 4: hipLaunchKernelGGL((mass_kernel<4>), grid, block, 0, st, imm, jmin, t, g); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

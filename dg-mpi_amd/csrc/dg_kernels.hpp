@@ -22,6 +22,7 @@ struct KTables {
     double BQ[2 * PMAX + 2][PMAX + 1];  // phi_i at the (2p+2)-point rule
     double xq[2 * PMAX + 2];        // (2p+2)-point GL nodes
     double wq[2 * PMAX + 2];        // (2p+2)-point GL weights
+    double xv[PMAX + 2];            // (p+2)-point GL nodes (volume and face points)
     int minv_diag;                  // 1: minv is diagonal (Legendre)
 };
 
@@ -37,6 +38,11 @@ struct KGeom {
     double gamma, R;
     int xcd_remap;                  // 1: map workgroups to elements XCD-contiguously
     int tile[3];                    // log2 brick edges of the stage-kernel traversal
+    // non-affine (warped) box: per-element trilinear geometry
+    int mapped;                     // 1: X and imm below are used
+    int exact_normals;              // 1: face normals at the flux points
+    const double* X;                // [E][8][3] node coordinates, node 4k+2j+i
+    const double* imm;              // [E][nb][nb] inverse mass matrices (symmetric)
 };
 
 // Initial condition / exact solution selector and data.
@@ -74,10 +80,15 @@ hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const 
                            const KGeom& g, hipStream_t st);
 hipError_t launch_init_state(int P, double time, double* U, const KTables* t, const KGeom& g,
                              const KIC& ic, hipStream_t st);
-// mode 0: per-element squared L2 error vs exact solution at time t -> out[E][5]
+// mode 0: per-element squared L2 error vs exact solution at time t and the
+//         element volume (sum of w |J|) -> out[E][6]
 // mode 1: per-element integrals (rho, rho u, rho v, rho w, rho E, KE) -> out[E][6]
 hipError_t launch_eval(int P, int mode, double time, const double* U, double* out,
                        const KTables* t, const KGeom& g, const KIC& ic, hipStream_t st);
+// mapped box: per-element mass matrices on the (2p+2)^3 rule, inverted in
+// place -> G.imm (written through `imm`); jmin[E] = smallest |J| at the
+// points (<= 0: invalid element)
+hipError_t launch_mass(int P, double* imm, double* jmin, const KTables* t, const KGeom& g, hipStream_t st);
 // per-element max(|u| + c) at the volume points -> out[E]
 hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables* t, const KGeom& g,
                             hipStream_t st);

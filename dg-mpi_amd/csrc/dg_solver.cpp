@@ -32,6 +32,8 @@ struct dg_solver {
     double* Ub = nullptr;
     double* acc = nullptr;             // RK accumulator
     double* work = nullptr;            // per-element reduction output
+    double* X = nullptr;               // mapped box: element node coordinates [E][8][3]
+    double* imm = nullptr;             // mapped box: inverse mass matrices [E][nb][nb]
     dg::KTables* dtab = nullptr;       // device copy of tab
     std::string err;
 };
@@ -154,12 +156,12 @@ void step_once(dg_solver* s, double dt) {
 void free_all(dg_solver* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
-    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work})
+    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work, s->X, s->imm})
         if (p) (void)hipFree(p);
     if (s->dtab) (void)hipFree(s->dtab);
     s->dtab = nullptr;
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
-    s->U0 = s->Ua = s->Ub = s->acc = s->work = nullptr;
+    s->U0 = s->Ua = s->Ub = s->acc = s->work = s->X = s->imm = nullptr;
     s->stream = s->own_stream = nullptr;
 }
 
@@ -198,6 +200,24 @@ int dg_create(const char* toml_text, dg_solver** out) {
         hip_check(hipMalloc(&s->dtab, sizeof(dg::KTables)), "hipMalloc(tables)");
         hip_check(hipMemcpy(s->dtab, &s->tab, sizeof(dg::KTables), hipMemcpyHostToDevice), "hipMemcpy(tables)");
         hip_check(hipMemsetAsync(s->U0, 0, bytes, s->stream), "hipMemset");
+        if (s->geo.mapped) {
+            // non-affine box: element nodes, then the per-element inverse mass
+            // matrices (precompute_matrix_helpers, src/solver/helpers.cpp:53-100)
+            const std::vector<double> X = dg::element_nodes(cfg);
+            hip_check(hipMalloc(&s->X, X.size() * sizeof(double)), "hipMalloc(X)");
+            hip_check(hipMemcpy(s->X, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(X)");
+            hip_check(hipMalloc(&s->imm, (size_t)s->geo.E * nb * nb * sizeof(double)), "hipMalloc(imm)");
+            s->geo.X = s->X;
+            s->geo.imm = s->imm;
+            hip_check(dg::launch_mass(s->P, s->imm, s->work, s->dtab, s->geo, s->stream), "mass kernel");
+            std::vector<double> jm(s->geo.E);
+            hip_check(hipMemcpyAsync(jm.data(), s->work, jm.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                     s->stream), "hipMemcpy");
+            hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+            double m = jm[0];
+            for (double v : jm) m = std::fmin(m, v);
+            if (!(m > 0.0)) throw dg::InputError("Mesh.warp folds an element (|J| <= 0 at a quadrature point)");
+        }
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     });
     if (rc != DG_OK) {
@@ -376,12 +396,19 @@ int dg_l2_error(dg_solver* s, double t, double err[5]) {
                   "hipMemcpy");
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
         // total volume: sum of element volumes on the (p+2)^3 rule
-        // (src/mesh/tools.cpp:27-47, src/post/main.cpp:49-54)
-        double wsum = 0.0;
-        for (int a = 0; a < s->P + 2; ++a)
-            for (int b = 0; b < s->P + 2; ++b)
-                for (int c = 0; c < s->P + 2; ++c) wsum += s->tab.w[a] * s->tab.w[b] * s->tab.w[c];
-        const double vol = (double)s->geo.E * (wsum * s->geo.jdet);
+        // (src/mesh/tools.cpp:27-47, src/post/main.cpp:49-54); on the mapped
+        // box the sum of w |J| that the eval kernel returns per element
+        // (the (2p+2)^3 rule, exact for trilinear |J| as the (p+2)^3 one is)
+        double vol;
+        if (s->geo.mapped) {
+            vol = csum(h, s->geo.E, 6, 5);
+        } else {
+            double wsum = 0.0;
+            for (int a = 0; a < s->P + 2; ++a)
+                for (int b = 0; b < s->P + 2; ++b)
+                    for (int c = 0; c < s->P + 2; ++c) wsum += s->tab.w[a] * s->tab.w[b] * s->tab.w[c];
+            vol = (double)s->geo.E * (wsum * s->geo.jdet);
+        }
         for (int v = 0; v < dg::NVAR; ++v) err[v] = std::pow(csum(h, s->geo.E, 6, v) / vol, 1. / 2);
     });
 }

@@ -111,6 +111,7 @@ KTables make_tables(const CaseConfig& c) {
     // (p+2)-point rule: volume and face quadrature (quadrature/tools.cpp:7-24,
     // quadrilateral.cpp:100-105)
     gl_rule(m, x, w);
+    for (int a = 0; a < m; ++a) t.xv[a] = x[a];
     for (int a = 0; a < m; ++a) {
         basis_1d(c.basis, p, x[a], v, g);
         for (int i = 0; i < n; ++i) {
@@ -185,6 +186,8 @@ KGeom make_geometry(const CaseConfig& c) {
     }
     g.gamma = c.gamma;
     g.R = c.gas_constant;
+    g.mapped = c.warp > 0.0 ? 1 : 0;
+    g.exact_normals = c.exact_normals ? 1 : 0;
     g.xcd_remap = (E % 8 == 0 && E >= 8 * 64) ? 1 : 0;
     // Stage-kernel traversal in bricks of up to 8^3 elements (edge: the
     // largest power of two <= 8 dividing the element count), so that the
@@ -205,6 +208,31 @@ KGeom make_geometry(const CaseConfig& c) {
         }
     }
     return g;
+}
+
+void box_node(const CaseConfig& c, long i, long j, long k, double* x) {
+    const long ijk[3] = {i, j, k};
+    double h[3], sn[3];
+    for (int d = 0; d < 3; ++d) {
+        h[d] = (c.upper[d] - c.lower[d]) / c.nelem[d];
+        sn[d] = std::sin(2.0 * M_PI * (double)(ijk[d] % c.nelem[d]) / (double)c.nelem[d]);
+    }
+    for (int d = 0; d < 3; ++d) {
+        x[d] = c.lower[d] + ijk[d] * h[d];
+        if (c.warp != 0.0) x[d] += c.warp * h[d] * (sn[(d + 1) % 3] * sn[(d + 2) % 3]);
+    }
+}
+
+std::vector<double> element_nodes(const CaseConfig& c) {
+    const long nx = c.nelem[0], ny = c.nelem[1], nz = c.nelem[2];
+    std::vector<double> X((size_t)nx * ny * nz * 24);
+    for (long iz = 0; iz < nz; ++iz)
+        for (long iy = 0; iy < ny; ++iy)
+            for (long ix = 0; ix < nx; ++ix) {
+                double* xe = X.data() + (size_t)(ix + nx * (iy + ny * iz)) * 24;
+                for (int n = 0; n < 8; ++n) box_node(c, ix + (n & 1), iy + ((n >> 1) & 1), iz + (n >> 2), xe + 3 * n);
+            }
+    return X;
 }
 
 KIC make_ic(const CaseConfig& c) {

@@ -26,4 +26,15 @@ KGeom make_geometry(const CaseConfig& c);
 
 KIC make_ic(const CaseConfig& c);
 
+// Node (i, j, k), i in [0, nelem[0]] etc., of the periodic box, displaced
+// by Mesh.warp:
+//   x = lo_x + i h_x + warp h_x sin(2 pi eta) sin(2 pi zeta)   (y, z cyclic)
+// with xi = (i mod nx)/nx etc.  Each component depends on the other two
+// indices only, so opposite box faces stay translates of each other.
+void box_node(const CaseConfig& c, long i, long j, long k, double* x);
+
+// Node coordinates of every element, [E][8][3], node n = 4k + 2j + i
+// (the reference's hex node order), element e = ix + nx (iy + ny iz).
+std::vector<double> element_nodes(const CaseConfig& c);
+
 }  // namespace dg

@@ -227,12 +227,16 @@ class Solver:
 
 def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(-5., -5., -5.),
               upper=(5., 5., 5.), stepper="RK4", timestep=None, cfl=None, end=1.0, gamma=1.4,
-              gas_constant=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0)):
-    """TOML text of a periodic-box case (keys of the reference input decks)."""
+              gas_constant=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0),
+              warp=0.0, normals="reference"):
+    """TOML text of a periodic-box case (keys of the reference input decks;
+    warp > 0 gives the non-affine box)."""
     st = f"timestep = {timestep!r}" if timestep is not None else f"cfl = {cfl!r}"
     arr = lambda v: "[" + ", ".join(repr(float(x)) for x in v) + "]"
     return f"""[Mesh]
 type = "PeriodicBox"
+warp = {float(warp)!r}
+normals = "{normals}"
 nelem = [{nelem[0]}, {nelem[1]}, {nelem[2]}]
 lower = {arr(lower)}
 upper = {arr(upper)}

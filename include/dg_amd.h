@@ -67,7 +67,10 @@ typedef struct {
 
 /* Parse a TOML case (the keys of src/exec/main.cpp:38-83,
  * src/solver/base.cpp:20-67, src/physics/base/base.cpp:24-32,
- * src/numerics/numerics_data.cpp:229-300; mesh keys: see INTEGRATION.md),
+ * src/numerics/numerics_data.cpp:229-300; in place of Mesh.file the mesh
+ * keys are type = "PeriodicBox", nelem, lower, upper, and optionally
+ * warp (periodic node displacement, element widths) and normals =
+ * "reference" | "exact"),
  * build the periodic mesh and the reference-element operators
  * (Solver<dim>::Solver + precompute_matrix_helpers,
  * src/solver/base.cpp:20-111) and allocate device state on the current

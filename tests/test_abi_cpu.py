@@ -53,6 +53,8 @@ BASE = dg_amd.case_toml(order=2, nelem=(2, 2, 2), timestep=0.01)
     (("nelem = [2, 2, 2]", "nelem = [2, 0, 2]"), 1, "positive integers"),
     (("data = [1.0, 1.0, 1.0, 1.0, 5.0]", "data = [1.0, 1.0]"), 1, "needs 5 values"),
     (("[Numerics]", "[Numerics\n"), 1, "TOML parse error"),
+    (("warp = 0.0", "warp = 0.3"), 1, "Mesh.warp must lie in"),
+    (("normals = \"reference\"", "normals = \"average\""), 1, "Mesh.normals"),
 ])
 def test_input_errors(edit, status, needle):
     text = BASE.replace(*edit)
@@ -60,6 +62,12 @@ def test_input_errors(edit, status, needle):
     rc, msg = _create(text)
     assert rc == status, msg
     assert needle in msg, msg
+
+
+def test_warped_order_limit():
+    text = BASE.replace("warp = 0.0", "warp = 0.1").replace("order = 2", "order = 5")
+    rc, msg = _create(text)
+    assert rc == 4 and "order > 4" in msg, msg
 
 
 def test_missing_keys():
