@@ -40,11 +40,11 @@ def alg_flops_per_element(p):
     return 2 * (4 * V * S + 12 * V * T) + 60 * m ** 3 + 300 * m ** 2 + 4 * V * n ** 3
 
 
-def case_toml(ne, p=P, end=1.0):
+def case_toml(ne, p=P, end=1.0, warp=0.0):
     import dg_amd
     return dg_amd.case_toml(order=p, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne),
                             lower=(-5., -5., -5.), upper=(5., 5., 5.), stepper="RK4", cfl=0.1, end=end,
-                            ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0))
+                            ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0), warp=warp)
 
 
 def measured_traffic():
@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ne", type=int, default=NE, help=argparse.SUPPRESS)
+    # other orders / the warped box: measurements beside the headline line
+    ap.add_argument("--order", type=int, default=P, help=argparse.SUPPRESS)
+    ap.add_argument("--warp", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -103,14 +106,14 @@ def main():
     from dg_amd import hip
 
     assert hip.device_count() > 0, "bench.py needs a GPU"
-    ne = args.ne
-    s = dg_amd.Solver(case_toml(ne))
+    ne, order = args.ne, args.order
+    s = dg_amd.Solver(case_toml(ne, order, warp=args.warp))
     stream = hip.Stream()                 # default priority
     s.set_stream(stream.handle)
     s.init_state_from_fcn()
     dt = s.timestep
     E = ne ** 3
-    nb = (P + 1) ** 3
+    nb = (order + 1) ** 3
     dof = E * nb
 
     s.take_time_step(dt, args.warmup)
@@ -134,7 +137,7 @@ def main():
 
     bytes_per_launch = ALG_BYTES_PER_DOF * dof
     achieved = bytes_per_launch / t_stage / 1e9
-    flops_launch = alg_flops_per_element(P) * E
+    flops_launch = alg_flops_per_element(order) * E
     out = {
         "metric": METRIC,
         "value": value,
@@ -148,12 +151,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (z-extruded isentropic vortex, generated on the GPU)",
-        "config": {"workload": f"euler3d_periodic_box_{ne}^3_p3_legendre_roe_rk4_cfl0.1",
-                   "elements": E, "order": P, "dof": dof, "variables": 5, "flux": "Roe",
+        "config": {"workload": f"euler3d_periodic_box_{ne}^3_p{order}_legendre_roe_rk4_cfl0.1"
+                               + (f"_warp{args.warp}" if args.warp else ""),
+                   "elements": E, "order": order, "dof": dof, "variables": 5, "flux": "Roe",
                    "basis": "LegendreHex", "stepper": "RK4", "dt": dt, "domain": "[-5,5]^3"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic()},
-        "kernel": {"name": "stage_kernel<3,Roe>", "avg_launch_ms": ms_per_step / 4,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if (order == P and ne == NE and not args.warp) else None},
+        "kernel": {"name": f"stage_kernel<{order},Roe{',mapped' if args.warp else ''}>",
+                   "avg_launch_ms": ms_per_step / 4,
                    "launches": 4 * args.steps, "alg_bytes_per_launch": bytes_per_launch,
                    "alg_flops_per_launch": flops_launch,
                    "fp64_tflops": flops_launch / t_stage / 1e12,

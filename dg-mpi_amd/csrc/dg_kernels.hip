@@ -39,16 +39,17 @@ namespace {
 
 constexpr int V = NVAR;
 constexpr int WAVE = 64;
-// Threads that cooperate on one element in the fused stage kernel: four
-// wavefronts from p = 3 on (measured at 64^3 p = 3: 64 threads 3.47 ms per
-// stage, 128: 2.48, 192: 2.26, 256: 2.12, 320: 3.33), two below, where an
-// element has fewer (line, variable) tasks.  DG_STAGE_TEAM overrides.
+// Threads that cooperate on one element in the fused stage kernel, by
+// order (measured, 64^3 Legendre Roe RK4, ms per stage for 64 / 128 / 256 /
+// 384 / 512 threads: p=1 0.70 / 0.84 / 1.18, p=2 1.51 / 1.30 / 1.58,
+// p=3 3.47 / 2.48 / 2.12 (320: 3.33), p=4 10.0 / 6.21 / 4.54 / 5.41 / 5.31,
+// p=5 20.7 / 11.4 / 8.00 / 12.2 / 6.79).  DG_STAGE_TEAM overrides.
 template <int P>
 constexpr int stage_team() {
 #ifdef DG_STAGE_TEAM
     return DG_STAGE_TEAM;
 #else
-    return P <= 2 ? 128 : 256;
+    return P == 1 ? 64 : (P == 2 ? 128 : (P <= 4 ? 256 : 512));
 #endif
 }
 
@@ -1278,8 +1279,8 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
         LT.mdiag[lane] = T.minv[lane][lane];
     }
     if constexpr (MAPPED) {
-        if (lane < 96) {
-            const int which = lane / 24;
+        team_for<96, TS>(lane, [&](int t) {
+            const int which = t / 24;
             unsigned el = e;
             if (which > 0) {
                 int ec[3];
@@ -1287,8 +1288,8 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
                 ec[which - 1] -= 1;
                 el = elem_at(ec[0], ec[1], ec[2], G);
             }
-            XL[lane] = G.X[(size_t)el * 24 + lane % 24];
-        }
+            XL[t] = G.X[(size_t)el * 24 + t % 24];
+        });
     }
     PF pf;
     pf.load_block(A.Ust, e, lane);
