@@ -878,23 +878,8 @@ static void build_faces(orc_solver* s) {
 static void build_normals(orc_solver* s) {
     int nqf = s->nqf;
     s->normals = xcalloc((size_t)s->F * nqf * 3, sizeof(double));
-    /* LagrangeQuad gorder 1 gradients at (pts(0,iq,0), pts(0,iq,1)):
-     * src/numerics/basis/tools.cpp:180-213 */
-    double* fg = xcalloc((size_t)nqf * 4 * 2, sizeof(double));
     double xn[2];
     orc_equidistant_nodes(-1., 1., 2, xn);
-    for (int q = 0; q < nqf; ++q) {
-        double vx[2], vy[2], gx[2], gy[2];
-        double a = s->fq_pts[q * 3 + 0], b = s->fq_pts[q * 3 + 1];
-        if (s->prm.exact_normals) { a = s->fq_lp[q * 2 + 0]; b = s->fq_lp[q * 2 + 1]; }
-        lagrange_val_1d(a, xn, 2, vx); lagrange_val_1d(b, xn, 2, vy);
-        lagrange_grad_1d(a, xn, 2, gx); lagrange_grad_1d(b, xn, 2, gy);
-        for (int j = 0; j < 2; ++j)
-            for (int i = 0; i < 2; ++i) {
-                fg[(q * 4 + j * 2 + i) * 2 + 0] = gx[i] * vy[j];
-                fg[(q * 4 + j * 2 + i) * 2 + 1] = vx[i] * gy[j];
-            }
-    }
     #pragma omp parallel for schedule(static)
     for (long f = 0; f < s->F; ++f) {
         /* one rank: the left element is always local, sign = +1 */
@@ -908,17 +893,34 @@ static void build_normals(orc_solver* s) {
         for (int d = 0; d < 3; ++d)
             for (int n = 0; n < 4; ++n) fc[n * 3 + d] = X[lf[n] * 3 + d];
         for (int q = 0; q < nqf; ++q) {
+            /* LagrangeQuad gorder-1 gradients (src/numerics/basis/tools.cpp:
+             * 180-213) at the reference's evaluation point, the face-0
+             * points' first two coordinates of index q (basis.cpp:324-338);
+             * exact_normals: at the left trace's own point qidxL[q] */
+            double a = s->fq_pts[q * 3 + 0], bq = s->fq_pts[q * 3 + 1];
+            if (s->prm.exact_normals) {
+                const int k = s->qidxL[f * nqf + q];
+                a = s->fq_lp[k * 2 + 0];
+                bq = s->fq_lp[k * 2 + 1];
+            }
+            double vx[2], vy[2], gx[2], gy[2], fg[8];
+            lagrange_val_1d(a, xn, 2, vx); lagrange_val_1d(bq, xn, 2, vy);
+            lagrange_grad_1d(a, xn, 2, gx); lagrange_grad_1d(bq, xn, 2, gy);
+            for (int j = 0; j < 2; ++j)
+                for (int i = 0; i < 2; ++i) {
+                    fg[(j * 2 + i) * 2 + 0] = gx[i] * vy[j];
+                    fg[(j * 2 + i) * 2 + 1] = vx[i] * gy[j];
+                }
             double xg[6];   /* [3][2] = fc^T fg */
-            gemm(3, 2, 4, 1., fc, 1, 3, fg + (size_t)q * 8, 2, 1, 0., xg, 2, 1);
-            double a[3] = {xg[0], xg[2], xg[4]}, b[3] = {xg[1], xg[3], xg[5]};
+            gemm(3, 2, 4, 1., fc, 1, 3, fg, 2, 1, 0., xg, 2, 1);
+            double av[3] = {xg[0], xg[2], xg[4]}, b[3] = {xg[1], xg[3], xg[5]};
             double* N = s->normals + ((size_t)f * nqf + q) * 3;
-            N[0] = a[1] * b[2] - a[2] * b[1];
-            N[1] = a[2] * b[0] - a[0] * b[2];
-            N[2] = a[0] * b[1] - a[1] * b[0];
+            N[0] = av[1] * b[2] - av[2] * b[1];
+            N[1] = av[2] * b[0] - av[0] * b[2];
+            N[2] = av[0] * b[1] - av[1] * b[0];
             for (int d = 0; d < 3; ++d) N[d] = sign * N[d];
         }
     }
-    free(fg);
 }
 
 /* inverse mass matrices: src/solver/helpers.cpp:53-100 with
@@ -1001,20 +1003,7 @@ static void build_face_helpers(orc_solver* s) {
     s->fq_lp = lp;
 }
 
-orc_solver* orc_create(const orc_params* prm) {
-    if (prm->order < 1) die("order must be >= 1");
-    if (prm->basis == ORC_LEGENDRE_HEX && prm->order > 7) die("LegendreHex supports p <= 7");
-    for (int d = 0; d < 3; ++d) if (prm->nelem[d] < 1) die("bad element count");
-    orc_solver* s = xcalloc(1, sizeof(orc_solver));
-    s->prm = *prm;
-    s->p = prm->order; s->n = s->p + 1; s->nb = s->n * s->n * s->n;
-    s->E = (long)prm->nelem[0] * prm->nelem[1] * prm->nelem[2];
-    build_mesh(s);
-    build_imm(s);
-    build_volume_helpers(s);
-    build_face_helpers(s);
-    build_faces(s);
-    build_normals(s);
+static void alloc_state(orc_solver* s) {
     size_t E = s->E, nb = s->nb, nq = s->nq, nqf = s->nqf, F = s->F;
     s->Uc = xcalloc(E * nb * NS, sizeof(double));
     s->res = xcalloc(E * nb * NS, sizeof(double));
@@ -1031,6 +1020,61 @@ orc_solver* orc_create(const orc_params* prm) {
     s->dU4 = xcalloc(E * nb * NS, sizeof(double));
     s->Uhold = xcalloc(E * nb * NS, sizeof(double));
     s->time = 0.0;
+}
+
+orc_solver* orc_create(const orc_params* prm) {
+    if (prm->order < 1) die("order must be >= 1");
+    if (prm->basis == ORC_LEGENDRE_HEX && prm->order > 7) die("LegendreHex supports p <= 7");
+    for (int d = 0; d < 3; ++d) if (prm->nelem[d] < 1) die("bad element count");
+    orc_solver* s = xcalloc(1, sizeof(orc_solver));
+    s->prm = *prm;
+    s->p = prm->order; s->n = s->p + 1; s->nb = s->n * s->n * s->n;
+    s->E = (long)prm->nelem[0] * prm->nelem[1] * prm->nelem[2];
+    build_mesh(s);
+    build_imm(s);
+    build_volume_helpers(s);
+    build_face_helpers(s);
+    build_faces(s);
+    build_normals(s);
+    alloc_state(s);
+    return s;
+}
+
+/* A mesh given as the reference's mesh datasets (src/mesh/mesh.cpp:79-175):
+ * NodeCoords [nnodes][3], Elem2Nodes [E][8] (node 4k + 2j + i), and the
+ * IFaceData 6-tuples [F][6] = (elemL, faceL, orientL, elemR, faceR, orientR);
+ * one rank.  prm's nelem is not used; lo/hi still give the period of the
+ * exact solutions that wrap. */
+orc_solver* orc_create_mesh(const orc_params* prm, long nnodes, const double* coords, long E,
+                            const long* e2n, long F, const int* iface6) {
+    if (prm->order < 1) die("order must be >= 1");
+    if (prm->basis == ORC_LEGENDRE_HEX && prm->order > 7) die("LegendreHex supports p <= 7");
+    orc_solver* s = xcalloc(1, sizeof(orc_solver));
+    s->prm = *prm;
+    s->p = prm->order; s->n = s->p + 1; s->nb = s->n * s->n * s->n;
+    s->E = E;
+    s->nnodes = nnodes;
+    s->node_coords = xcalloc((size_t)nnodes * 3, sizeof(double));
+    memcpy(s->node_coords, coords, sizeof(double) * nnodes * 3);
+    s->e2n = xcalloc((size_t)E * 8, sizeof(long));
+    memcpy(s->e2n, e2n, sizeof(long) * E * 8);
+    build_imm(s);
+    build_volume_helpers(s);
+    build_face_helpers(s);
+    s->F = F;
+    s->ifaces = xcalloc((size_t)F * 8, sizeof(unsigned));
+    s->qidxL = xcalloc((size_t)F * s->nqf, sizeof(int));
+    s->qidxR = xcalloc((size_t)F * s->nqf, sizeof(int));
+    for (long f = 0; f < F; ++f) {
+        const int* r = iface6 + f * 6;
+        unsigned* t = s->ifaces + f * 8;
+        t[0] = 0; t[1] = (unsigned)r[0]; t[2] = (unsigned)r[1]; t[3] = (unsigned)r[2];
+        t[4] = 0; t[5] = (unsigned)r[3]; t[6] = (unsigned)r[4]; t[7] = (unsigned)r[5];
+        orc_face_pts_order(r[2], s->nqf, s->qidxL + f * s->nqf);
+        orc_face_pts_order(r[5], s->nqf, s->qidxR + f * s->nqf);
+    }
+    build_normals(s);
+    alloc_state(s);
     return s;
 }
 

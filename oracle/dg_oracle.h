@@ -89,6 +89,8 @@ void   orc_ic_eval(const orc_params* prm, const double* x, double t, double* U);
 void orc_box_node(const orc_params* prm, long i, long j, long k, double* x);
 
 orc_solver* orc_create(const orc_params* prm);
+orc_solver* orc_create_mesh(const orc_params* prm, long nnodes, const double* coords, long E,
+                            const long* e2n, long F, const int* iface6);
 void   orc_destroy(orc_solver* s);
 long   orc_num_elems(const orc_solver* s);
 int    orc_num_basis(const orc_solver* s);

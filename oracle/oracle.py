@@ -62,6 +62,9 @@ def lib():
             "orc_flux_numerical": (None, [C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
             "orc_ic_eval": (None, [C.POINTER(Params), _dp, C.c_double, _dp]),
             "orc_create": (C.c_void_p, [C.POINTER(Params)]),
+            "orc_create_mesh": (C.c_void_p, [C.POINTER(Params), C.c_long, _dp, C.c_long,
+                                             np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                             C.c_long, _ip]),
             "orc_destroy": (None, [C.c_void_p]),
             "orc_num_elems": (C.c_long, [C.c_void_p]),
             "orc_num_basis": (C.c_int, [C.c_void_p]),
@@ -179,9 +182,18 @@ def ic_eval(params, x, t=0.0):
 class Solver:
     """Oracle solver on a periodic box (reference semantics)."""
 
-    def __init__(self, params):
+    def __init__(self, params, mesh=None):
+        """mesh: (NodeCoords [n][3], Elem2Nodes [E][8], IFaceData [F][6]) in
+        the reference's mesh-file layout; None: the periodic box of params."""
         self.params = params
-        self.h = lib().orc_create(C.byref(params))
+        if mesh is None:
+            self.h = lib().orc_create(C.byref(params))
+        else:
+            xyz = np.ascontiguousarray(mesh[0], dtype=np.float64)
+            e2n = np.ascontiguousarray(mesh[1], dtype=np.int64)
+            ifd = np.ascontiguousarray(mesh[2], dtype=np.int32)
+            self.h = lib().orc_create_mesh(C.byref(params), len(xyz), xyz.ravel(), len(e2n), e2n.ravel(),
+                                           len(ifd), ifd.ravel())
         self.E = lib().orc_num_elems(self.h)
         self.nb = lib().orc_num_basis(self.h)
         self.F = lib().orc_num_ifaces(self.h)
