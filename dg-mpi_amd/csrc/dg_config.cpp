@@ -221,9 +221,10 @@ int pick(const std::string& str, const std::vector<std::string>& valid) {
 
 }  // namespace
 
-CaseConfig parse_case(const std::string& toml_text) {
+CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     TomlDoc d = parse_toml(toml_text);
     CaseConfig c;
+    c.mesh_input = mesh_input;
 
     // [Physics]
     std::string phys = get_str(d, "Physics", "name", true, "");
@@ -294,11 +295,32 @@ CaseConfig parse_case(const std::string& toml_text) {
                          "(src/physics/euler/functions.cpp:107)");
 
     // [Mesh]
+    if (mesh_input) {
+        // the mesh comes with the call (dg_create_mesh); lower / upper, if
+        // given, are the periodic cell of the advected exact solutions
+        std::vector<double> lo = get_num_array(d, "Mesh", "lower", false);
+        std::vector<double> hi = get_num_array(d, "Mesh", "upper", false);
+        if (lo.size() == 3 && hi.size() == 3)
+            for (int i = 0; i < 3; ++i) {
+                c.lower[i] = lo[i];
+                c.upper[i] = hi[i];
+            }
+        const std::string nrm = get_str(d, "Mesh", "normals", false, "reference");
+        if (nrm == "exact") c.exact_normals = true;
+        else if (nrm != "reference") throw InputError("Mesh.normals must be \"reference\" or \"exact\"");
+        if (get_num(d, "Mesh", "warp", false, 0.0) != 0.0)
+            throw InputError("Mesh.warp applies to the periodic box only");
+        if (c.timestep <= 0.0) throw InputError("Stepper.timestep is required with a mesh (Stepper.cfl needs the box)");
+        if (c.order > 4)
+            throw UnsupportedError("meshes with order > 4: the per-element mass matrix inverse of the "
+                                   "non-affine path is formed in LDS and holds at most (4+1)^3 basis functions");
+        return c;
+    }
     std::string mtype = get_str(d, "Mesh", "type", false, "");
     if (mtype.empty()) {
         if (find(d, "Mesh", "file"))
-            throw UnsupportedError("Mesh.file (HDF5 gmsh meshes) is not read by this path; "
-                                   "give Mesh.type = \"PeriodicBox\" with Mesh.nelem/lower/upper");
+            throw UnsupportedError("Mesh.file: HDF5 mesh files are read by dg_create_from_mesh_file "
+                                   "(libdg_amd_h5); dg_create takes Mesh.type = \"PeriodicBox\"");
         missing("Mesh", "type");
     }
     if (mtype != "PeriodicBox") throw InputError("Mesh.type must be \"PeriodicBox\"");

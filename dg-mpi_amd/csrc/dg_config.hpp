@@ -55,6 +55,7 @@ struct CaseConfig {
     double lower[3] = {0, 0, 0}, upper[3] = {1, 1, 1};
     double warp = 0.0;
     bool exact_normals = false;
+    bool mesh_input = false;   // the mesh came with dg_create_mesh
     // [Stepper] (src/solver/base.cpp:46-57)
     Stepper stepper = Stepper::FE;
     double timestep = 0.0;     // 0 => derived from cfl
@@ -69,6 +70,7 @@ struct CaseConfig {
     std::vector<double> ic_data;
 };
 
-CaseConfig parse_case(const std::string& toml_text);
+// mesh_input: the mesh is given with the call (no box keys required)
+CaseConfig parse_case(const std::string& toml_text, bool mesh_input = false);
 
 }  // namespace dg

@@ -696,6 +696,41 @@ __device__ __forceinline__ int bidx_d(int d, int r, int t1, int t2) {
     return r * sr + t1 * s1 + t2 * s2;
 }
 
+// Reference face ids (shape.h:80-86: 0 z-, 1 y-, 2 x+, 3 y+, 4 x-, 5 z+)
+// as (normal direction d, side: 1 the + face), from packed bit tables.
+__device__ __forceinline__ int face_dir(int f) { return (2118 >> (2 * f)) & 3; }
+__device__ __forceinline__ int face_side(int f) { return (44 >> f) & 1; }
+
+// Face record of an element face (slot d*2 + side) on the general path:
+// frec[2 slot] = neighbour element, frec[2 slot + 1] = neighbour face id
+// (bits 0-2) | map code (bits 3-5) | this element is the face's left one
+// (bit 6) | left face id * 8 + left orientation (bits 8-13).  The map
+// code takes this face's point (a1, a2) to the neighbour face's (b1, b2):
+// swap (bit 0), then reverse b1 (bit 1), reverse b2 (bit 2).
+__device__ __forceinline__ void map_point(int code, int m, int a1, int a2, int& b1, int& b2) {
+    const int c1 = (code & 1) ? a2 : a1, c2 = (code & 1) ? a1 : a2;
+    b1 = (code & 2) ? m - 1 - c1 : c1;
+    b2 = (code & 4) ? m - 1 - c2 : c2;
+}
+
+// Outward area-weighted normal of face (d, side) of an element at the
+// face's tangential reference coordinates (u1, u2): the cofactor row d on
+// xi_d = +-1, negated on the - face.
+__device__ __forceinline__ void face_normal_gen(const double* X, int d, int side, double u1, double u2,
+                                                double* N) {
+    const double x = side ? 1. : -1.;
+    double a[3][3];
+    if (d == 0) trilinear(X, x, u1, u2, a);
+    else if (d == 1) trilinear(X, u1, x, u2, a);
+    else trilinear(X, u1, u2, x, a);
+    cross3(a[(d + 1) % 3], a[(d + 2) % 3], N);
+    if (!side) {
+        N[0] = -N[0];
+        N[1] = -N[1];
+        N[2] = -N[2];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Face phase: traces (own and neighbour) on all six faces, Riemann fluxes,
 // lift, accumulated into sR.  The neighbour slab uses exactly the
@@ -705,7 +740,7 @@ __device__ __forceinline__ int bidx_d(int d, int r, int t1, int t2) {
 template <int P, int FLUX, int TS, bool MAPPED = false>
 __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, const KGeom& G, int lane,
                                            const double* sU, double* sR, double* bankA, double* bankB,
-                                           const double* XL = nullptr) {
+                                           const double* XL = nullptr, unsigned eid = 0) {
     using Dm = Dims<P>;
     using L = StageLDS<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, NNN = Dm::NNN;
@@ -775,21 +810,31 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
             UL[s] = side == 0 ? nbv[s] : own[s];
             UR[s] = side == 0 ? own[s] : nbv[s];
         }
+        double sgn = side == 0 ? 1. : -1.;   // the box: the + face's element is the left one
         if constexpr (MAPPED) {
-            // normal of the face's left element (own + face, or the + face
-            // of the neighbour across the - face), where the reference
-            // evaluates it: at each face's (s, t) swapped (basis.cpp:332;
-            // s, t are (xi_y, xi_z), (-xi_x, xi_z), (xi_x, xi_y) for the +x,
-            // +y, +z faces, shape.cpp:296-323), or at the flux point
-            const double* XLe = XL + (side ? 0 : (1 + d) * 24);
-            double u1 = LT.xv[a1], u2 = LT.xv[a2];
-            if (!G.exact_normals) {
-                u1 = d == 1 ? -LT.xv[a2] : LT.xv[a2];
-                u2 = d == 1 ? -LT.xv[a1] : LT.xv[a1];
+            // general connectivity: the neighbour's trace sits at its own
+            // face point (b1, b2); the flux is evaluated with the face's
+            // left state first and the left element's normal, at the point
+            // where the reference evaluates it (table nev: each left face's
+            // (s, t) swapped, basis.cpp:332; or the flux point itself)
+            const unsigned meta = G.frec[((size_t)eid * 6 + (d * 2 + side)) * 2 + 1];
+            const bool isL = (meta >> 6) & 1;
+            int b1, b2;
+            map_point((meta >> 3) & 7, M, a1, a2, b1, b2);
+            const double* nbp = f2 + (((d * 4 + 2 + side) * M + b2) * M + b1) * V;
+#pragma unroll
+            for (int s = 0; s < V; ++s) {
+                UL[s] = isL ? own[s] : nbp[s];
+                UR[s] = isL ? nbp[s] : own[s];
             }
+            const int lfo = (meta >> 8) & 63, fL = lfo >> 3;
+            const int c1 = isL ? a1 : b1, c2 = isL ? a2 : b2;
+            const double* ev = G.nev + ((size_t)lfo * MM + c2 * M + c1) * 2;
             double Nv[3];
-            face_normal(XLe, d, u1, u2, Nv);
+            face_normal_gen(XL + (isL ? 0 : (1 + d * 2 + side) * 24), face_dir(fL), face_side(fL), ev[0], ev[1],
+                            Nv);
             riemann_n<FLUX>(G.gamma, UL, UR, Nv, F);
+            sgn = isL ? -1. : 1.;
         } else {
             if (d == 0) riemann<FLUX, 0>(G.gamma, UL, UR, G.farea[0], F);
             else if (d == 1) riemann<FLUX, 1>(G.gamma, UL, UR, G.farea[1], F);
@@ -798,10 +843,7 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
         const double wq = LT.w[a1] * LT.w[a2];
         double* o = fh + (((d * 2 + side) * M + a2) * M + a1) * V;
 #pragma unroll
-        for (int s = 0; s < V; ++s) {
-            const double fw = F[s] * wq;
-            o[s] = side == 0 ? fw : -fw;
-        }
+        for (int s = 0; s < V; ++s) o[s] = sgn * (F[s] * wq);
     });
     sync();
     // 5. lift: transpose of the interpolation, along t2 then t1
@@ -1141,6 +1183,7 @@ struct StageLoads {
     double own[RU];
     double nl[RN][N];
     double u0[RU], ac[RU];
+    int nside[RN];          // general path: side of the neighbour's face
 
     __device__ __forceinline__ void load_block(const double* Ust, unsigned e, int lane) {
         const double* src = Ust + (size_t)e * NB;
@@ -1168,6 +1211,28 @@ struct StageLoads {
             }
         }
     }
+    // general connectivity: the neighbour and its face come from the face
+    // records; the lines run along the neighbour's own normal direction and
+    // the slab is kept in the neighbour's tangential frame
+    __device__ __forceinline__ void load_neighbours_gen(const double* Ust, unsigned e, const KGeom& G,
+                                                        int lane) {
+#pragma unroll
+        for (int r = 0; r < RN; ++r) {
+            const int t = r * TS + lane;
+            nside[r] = 0;
+            if (t < NT) {
+                const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
+                const unsigned* rec = G.frec + ((size_t)e * 6 + ds) * 2;
+                const unsigned en = rec[0];
+                const int fn = rec[1] & 7;
+                const int dn = face_dir(fn);
+                nside[r] = face_side(fn);
+                const double* Un = Ust + (size_t)en * NB;
+#pragma unroll
+                for (int q = 0; q < N; ++q) nl[r][q] = Un[bidx_d<N>(dn, q, t1, t2) * V + s];
+            }
+        }
+    }
     __device__ __forceinline__ void load_rk(const StageArgs& A, unsigned e, int lane) {
         const size_t base = (size_t)e * NB;
         const bool need_acc = A.mode == MODE_RK_MID || A.mode == MODE_RK_LAST;
@@ -1181,6 +1246,22 @@ struct StageLoads {
     __device__ __forceinline__ void clear_rk() {
 #pragma unroll
         for (int r = 0; r < RU; ++r) u0[r] = ac[r] = 0.;
+    }
+    // general path: the trace of the neighbour's face side nside
+    __device__ __forceinline__ void store_neighbour_slabs_gen(const LaneTab& LT, double* slab, int lane) const {
+#pragma unroll
+        for (int r = 0; r < RN; ++r) {
+            const int t = r * TS + lane;
+            if (t < NT) {
+                const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
+                const int d = ds >> 1, side = ds & 1;
+                const double* phi = nside[r] ? LT.phip : LT.phim;
+                double acc = 0.;
+#pragma unroll
+                for (int q = 0; q < N; ++q) acc = fma(phi[q], nl[r][q], acc);
+                slab[((d * 4 + 2 + side) * NN + t2 * N + t1) * V + s] = acc;
+            }
+        }
     }
     // neighbour slabs from the prefetched lines: the neighbour across the -
     // face contributes its + trace and vice versa; same fma chain as the
@@ -1256,9 +1337,9 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     constexpr int NB = Dm::NNN * V;
     __shared__ double lds[L::TOTAL];
     __shared__ LaneTab LT;
-    // mapped: node coordinates of the element and of its -x, -y, -z
-    // neighbours (the left elements of its - faces)
-    __shared__ double XL[MAPPED ? 96 : 1];
+    // general path: node coordinates of the element and of its six face
+    // neighbours (slot d*2 + side), for the left elements' normals
+    __shared__ double XL[MAPPED ? 168 : 1];
     double* sR = lds;
     double* bankA = lds + L::R;
     double* bankB = bankA + L::A;
@@ -1279,21 +1360,16 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
         LT.mdiag[lane] = T.minv[lane][lane];
     }
     if constexpr (MAPPED) {
-        team_for<96, TS>(lane, [&](int t) {
+        team_for<168, TS>(lane, [&](int t) {
             const int which = t / 24;
-            unsigned el = e;
-            if (which > 0) {
-                int ec[3];
-                elem_ijk(e, G, ec);
-                ec[which - 1] -= 1;
-                el = elem_at(ec[0], ec[1], ec[2], G);
-            }
+            const unsigned el = which == 0 ? e : G.frec[((size_t)e * 6 + (which - 1)) * 2];
             XL[t] = G.X[(size_t)el * 24 + t % 24];
         });
     }
     PF pf;
     pf.load_block(A.Ust, e, lane);
-    pf.load_neighbours(A.Ust, e, G, lane);
+    if constexpr (MAPPED) pf.load_neighbours_gen(A.Ust, e, G, lane);
+    else pf.load_neighbours(A.Ust, e, G, lane);
     if (rk) pf.load_rk(A, e, lane);
     else pf.clear_rk();
     sync();
@@ -1305,9 +1381,10 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
             sR[t] = 0.;
         }
     }
-    pf.store_neighbour_slabs(LT, bankB, lane);
+    if constexpr (MAPPED) pf.store_neighbour_slabs_gen(LT, bankB, lane);
+    else pf.store_neighbour_slabs(LT, bankB, lane);
     sync();
-    face_phase<P, FLUX, TS, MAPPED>(T, LT, G, lane, bankA, sR, bankA, bankB, XL);
+    face_phase<P, FLUX, TS, MAPPED>(T, LT, G, lane, bankA, sR, bankA, bankB, XL, e);
     volume_phase<P, TS, MAPPED>(T, LT, G, lane, sR, bankA, bankB, XL);
 
     // Epilogue with one store site per array; the value and the target are

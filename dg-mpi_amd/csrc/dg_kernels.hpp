@@ -38,11 +38,14 @@ struct KGeom {
     double gamma, R;
     int xcd_remap;                  // 1: map workgroups to elements XCD-contiguously
     int tile[3];                    // log2 brick edges of the stage-kernel traversal
-    // non-affine (warped) box: per-element trilinear geometry
-    int mapped;                     // 1: X and imm below are used
+    // general path (warped box, meshes from files): trilinear elements with
+    // face records instead of box neighbours
+    int mapped;                     // 1: the fields below are used
     int exact_normals;              // 1: face normals at the flux points
     const double* X;                // [E][8][3] node coordinates, node 4k+2j+i
     const double* imm;              // [E][nb][nb] inverse mass matrices (symmetric)
+    const unsigned* frec;           // [E][6][2] face records, slot d*2 + side
+    const double* nev;              // [48][(p+2)^2][2] normal evaluation points
 };
 
 // Initial condition / exact solution selector and data.

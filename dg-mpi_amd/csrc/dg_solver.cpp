@@ -34,6 +34,8 @@ struct dg_solver {
     double* work = nullptr;            // per-element reduction output
     double* X = nullptr;               // mapped box: element node coordinates [E][8][3]
     double* imm = nullptr;             // mapped box: inverse mass matrices [E][nb][nb]
+    unsigned* frec = nullptr;          // general path: face records [E][6][2]
+    double* nev = nullptr;             // general path: normal evaluation points
     dg::KTables* dtab = nullptr;       // device copy of tab
     std::string err;
 };
@@ -156,8 +158,11 @@ void step_once(dg_solver* s, double dt) {
 void free_all(dg_solver* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
-    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work, s->X, s->imm})
+    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work, s->X, s->imm, s->nev})
         if (p) (void)hipFree(p);
+    if (s->frec) (void)hipFree(s->frec);
+    s->frec = nullptr;
+    s->nev = nullptr;
     if (s->dtab) (void)hipFree(s->dtab);
     s->dtab = nullptr;
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
@@ -169,13 +174,54 @@ void free_all(dg_solver* s) {
 
 extern "C" {
 
-int dg_create(const char* toml_text, dg_solver** out) {
+}  // extern "C"
+
+namespace {
+
+struct MeshInput {
+    long nnodes = 0, E = 0, F = 0;
+    const double* coords = nullptr;
+    const long* e2n = nullptr;
+    const int* iface6 = nullptr;
+};
+
+int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
     if (out) *out = nullptr;
     dg_solver* s = nullptr;
     int rc = guarded(nullptr, [&] {
         need(toml_text, "toml_text");
         need(out, "out");
-        dg::CaseConfig cfg = dg::parse_case(toml_text);
+        dg::CaseConfig cfg = dg::parse_case(toml_text, mesh != nullptr);
+        if (mesh) {
+            if (mesh->E < 1 || mesh->nnodes < 1 || mesh->F < 1 || mesh->E > 0x7fffffffL)
+                throw dg::InputError("mesh: empty or too large");
+            need(mesh->coords, "coords");
+            need(mesh->e2n, "elem_nodes");
+            need(mesh->iface6, "iface_data");
+            for (long i = 0; i < mesh->E * 8; ++i)
+                if (mesh->e2n[i] < 0 || mesh->e2n[i] >= mesh->nnodes)
+                    throw dg::InputError("mesh: element node index out of range");
+            for (long i = 0; i < mesh->nnodes * 3; ++i)
+                if (!std::isfinite(mesh->coords[i])) throw dg::InputError("mesh: non-finite node coordinate");
+            cfg.nelem[0] = (int)mesh->E;
+            cfg.nelem[1] = cfg.nelem[2] = 1;
+        }
+        // host tables of the general path (warped box or a given mesh):
+        // element nodes, face records and normal points; mesh errors are
+        // reported here, before any device work
+        std::vector<double> X;
+        dg::FaceTables ft;
+        if (mesh) {
+            X.resize((size_t)mesh->E * 24);
+            for (long e = 0; e < mesh->E; ++e)
+                for (int n = 0; n < 8; ++n)
+                    for (int d = 0; d < 3; ++d)
+                        X[(size_t)e * 24 + n * 3 + d] = mesh->coords[mesh->e2n[e * 8 + n] * 3 + d];
+            ft = dg::mesh_face_tables(cfg, mesh->E, mesh->F, mesh->iface6);
+        } else if (cfg.warp > 0.0) {
+            X = dg::element_nodes(cfg);
+            ft = dg::box_face_tables(cfg);
+        }
         s = new dg_solver();
         s->cfg = cfg;
         s->P = cfg.order;
@@ -201,14 +247,22 @@ int dg_create(const char* toml_text, dg_solver** out) {
         hip_check(hipMemcpy(s->dtab, &s->tab, sizeof(dg::KTables), hipMemcpyHostToDevice), "hipMemcpy(tables)");
         hip_check(hipMemsetAsync(s->U0, 0, bytes, s->stream), "hipMemset");
         if (s->geo.mapped) {
-            // non-affine box: element nodes, then the per-element inverse mass
-            // matrices (precompute_matrix_helpers, src/solver/helpers.cpp:53-100)
-            const std::vector<double> X = dg::element_nodes(cfg);
+            // upload the general-path tables, then form the per-element
+            // inverse mass matrices (precompute_matrix_helpers,
+            // src/solver/helpers.cpp:53-100)
             hip_check(hipMalloc(&s->X, X.size() * sizeof(double)), "hipMalloc(X)");
             hip_check(hipMemcpy(s->X, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(X)");
+            hip_check(hipMalloc(&s->frec, ft.frec.size() * sizeof(unsigned)), "hipMalloc(frec)");
+            hip_check(hipMemcpy(s->frec, ft.frec.data(), ft.frec.size() * sizeof(unsigned), hipMemcpyHostToDevice),
+                      "hipMemcpy(frec)");
+            hip_check(hipMalloc(&s->nev, ft.nev.size() * sizeof(double)), "hipMalloc(nev)");
+            hip_check(hipMemcpy(s->nev, ft.nev.data(), ft.nev.size() * sizeof(double), hipMemcpyHostToDevice),
+                      "hipMemcpy(nev)");
             hip_check(hipMalloc(&s->imm, (size_t)s->geo.E * nb * nb * sizeof(double)), "hipMalloc(imm)");
             s->geo.X = s->X;
             s->geo.imm = s->imm;
+            s->geo.frec = s->frec;
+            s->geo.nev = s->nev;
             hip_check(dg::launch_mass(s->P, s->imm, s->work, s->dtab, s->geo, s->stream), "mass kernel");
             std::vector<double> jm(s->geo.E);
             hip_check(hipMemcpyAsync(jm.data(), s->work, jm.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -216,7 +270,7 @@ int dg_create(const char* toml_text, dg_solver** out) {
             hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
             double m = jm[0];
             for (double v : jm) m = std::fmin(m, v);
-            if (!(m > 0.0)) throw dg::InputError("Mesh.warp folds an element (|J| <= 0 at a quadrature point)");
+            if (!(m > 0.0)) throw dg::InputError("mesh: an element folds (|J| <= 0 at a quadrature point)");
         }
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     });
@@ -227,6 +281,24 @@ int dg_create(const char* toml_text, dg_solver** out) {
     }
     *out = s;
     return DG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_create(const char* toml_text, dg_solver** out) { return create(toml_text, nullptr, out); }
+
+int dg_create_mesh(const char* toml_text, long num_nodes, const double* node_coords, long num_elems,
+                   const long* elem_nodes, long num_ifaces, const int* iface_data, dg_solver** out) {
+    MeshInput m;
+    m.nnodes = num_nodes;
+    m.coords = node_coords;
+    m.E = num_elems;
+    m.e2n = elem_nodes;
+    m.F = num_ifaces;
+    m.iface6 = iface_data;
+    return create(toml_text, &m, out);
 }
 
 int dg_destroy(dg_solver* s) {

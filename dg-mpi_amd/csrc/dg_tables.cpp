@@ -186,7 +186,7 @@ KGeom make_geometry(const CaseConfig& c) {
     }
     g.gamma = c.gamma;
     g.R = c.gas_constant;
-    g.mapped = c.warp > 0.0 ? 1 : 0;
+    g.mapped = (c.warp > 0.0 || c.mesh_input) ? 1 : 0;
     g.exact_normals = c.exact_normals ? 1 : 0;
     g.xcd_remap = (E % 8 == 0 && E >= 8 * 64) ? 1 : 0;
     // Stage-kernel traversal in bricks of up to 8^3 elements (edge: the
@@ -207,6 +207,8 @@ KGeom make_geometry(const CaseConfig& c) {
                 for (int d = 0; d < 3; ++d) g.tile[d] = t[d];
         }
     }
+    // a given mesh is traversed in its own element order
+    if (c.mesh_input) g.tile[0] = g.tile[1] = g.tile[2] = 0;
     return g;
 }
 
@@ -233,6 +235,258 @@ std::vector<double> element_nodes(const CaseConfig& c) {
                 for (int n = 0; n < 8; ++n) box_node(c, ix + (n & 1), iy + ((n >> 1) & 1), iz + (n >> 2), xe + 3 * n);
             }
     return X;
+}
+
+// ---------------------------------------------------------------------------
+// Face conventions of the reference hexahedron (src/numerics/basis/
+// shape.cpp:236-428), restated for the face records of the general path
+// ---------------------------------------------------------------------------
+namespace {
+
+const int kFaceDir[6] = {2, 1, 0, 1, 0, 2};    // faces 0 z-, 1 y-, 2 x+, 3 y+, 4 x-, 5 z+
+const int kFaceSide[6] = {0, 0, 1, 1, 0, 1};
+// corner nodes of each face in the reference's cyclic order
+const int kFaceNodes[6][4] = {{0, 2, 3, 1}, {0, 1, 5, 4}, {1, 3, 7, 5}, {3, 2, 6, 7}, {2, 0, 4, 6}, {4, 5, 7, 6}};
+
+int face_id(int d, int side) {
+    return d == 0 ? (side ? 2 : 4) : (d == 1 ? (side ? 3 : 1) : (side ? 5 : 0));
+}
+
+void tangential(int d, int& t1, int& t2) {
+    t1 = d == 0 ? 1 : 0;
+    t2 = d == 2 ? 1 : 2;
+}
+
+// element coordinates of face parameter (s, t), orientation 0: bilinear
+// over the face's corners in cyclic order (shape.cpp:296-323)
+void face_map(int f, double s, double t, double* xi) {
+    const double xb = 0.5 * (s + 1.0), yb = 0.5 * (t + 1.0);
+    const double b[4] = {(1. - xb) * (1. - yb), xb * (1. - yb), xb * yb, (1. - xb) * yb};
+    for (int r = 0; r < 3; ++r) {
+        double v = 0.;
+        for (int j = 0; j < 4; ++j) {
+            const int n = kFaceNodes[f][j];
+            const double c = r == 0 ? 2. * (n & 1) - 1. : (r == 1 ? 2. * ((n >> 1) & 1) - 1. : 2. * (n >> 2) - 1.);
+            v += b[j] * c;
+        }
+        xi[r] = v;
+    }
+}
+
+// the eight face point orderings (shape.cpp:326-428)
+void face_pts_order(int orient, int npts, int* po) {
+    const int n = (int)std::lround(std::sqrt((double)npts)), np1 = npts + 1;
+    switch (orient) {
+    case 0:
+        for (int i = 0; i < npts; i++) po[i] = i;
+        break;
+    case 1: {
+        int c = n - 1;
+        for (int i = 0; i < npts; i++) {
+            po[i] = c;
+            c = (c + n) % np1;
+        }
+        break;
+    }
+    case 2:
+        for (int i = 0; i < npts; i++) po[i] = npts - 1 - i;
+        break;
+    case 3: {
+        int c = npts - n;
+        for (int i = 0; i < npts; i++) {
+            po[i] = c;
+            c -= n;
+            if (c < 0) c += np1;
+        }
+        break;
+    }
+    case 4: {
+        int c = 0;
+        for (int i = 0; i < npts; i++) {
+            po[i] = c;
+            c += n;
+            if (c >= npts) c -= npts - 1;
+        }
+        break;
+    }
+    case 5: {
+        int c = n - 1, k = 0;
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < n; j++) po[k++] = c--;
+            c += 2 * n;
+        }
+        break;
+    }
+    case 6: {
+        int c = npts - 1;
+        for (int i = 0; i < npts; i++) {
+            po[i] = c;
+            c -= n;
+            if (c < 0) c += npts - 1;
+        }
+        break;
+    }
+    case 7: {
+        int c = npts - n, k = 0;
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < n; j++) po[k++] = c++;
+            c -= 2 * n;
+        }
+        break;
+    }
+    default:
+        throw InputError("face orientation " + std::to_string(orient) + " is not 0..7");
+    }
+}
+
+// face f's grid point (a1, a2) (xi_t1 = x[a1], xi_t2 = x[a2]) -> canonical
+// point index k = j m + i of parameters (x[i], x[j]); and the inverse
+struct FaceGrid {
+    std::vector<int> g2c, c2g;   // [m*m]
+};
+
+FaceGrid face_grid(int f, int m, const double* x) {
+    FaceGrid g;
+    g.g2c.assign(m * m, -1);
+    g.c2g.assign(m * m, -1);
+    int t1, t2;
+    tangential(kFaceDir[f], t1, t2);
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) {
+            double xi[3];
+            face_map(f, x[i], x[j], xi);
+            int a1 = -1, a2 = -1;
+            for (int a = 0; a < m; ++a) {
+                if (std::fabs(xi[t1] - x[a]) < 1e-12) a1 = a;
+                if (std::fabs(xi[t2] - x[a]) < 1e-12) a2 = a;
+            }
+            if (a1 < 0 || a2 < 0) throw std::logic_error("face grid: point not on the tensor grid");
+            g.g2c[a2 * m + a1] = j * m + i;
+            g.c2g[j * m + i] = a2 * m + a1;
+        }
+    return g;
+}
+
+int map_code(const std::vector<int>& b_of_a, int m) {
+    for (int code = 0; code < 8; ++code) {
+        bool ok = true;
+        for (int a2 = 0; a2 < m && ok; ++a2)
+            for (int a1 = 0; a1 < m && ok; ++a1) {
+                const int c1 = (code & 1) ? a2 : a1, c2 = (code & 1) ? a1 : a2;
+                const int b1 = (code & 2) ? m - 1 - c1 : c1, b2 = (code & 4) ? m - 1 - c2 : c2;
+                ok = b_of_a[a2 * m + a1] == b2 * m + b1;
+            }
+        if (ok) return code;
+    }
+    throw InputError("IFaceData: a face pairing is not a symmetry of the face point grid");
+}
+
+std::vector<double> normal_points(const KTables& t, int m, bool exact) {
+    std::vector<double> nev((size_t)48 * m * m * 2);
+    std::vector<int> po(m * m), inv(m * m);
+    for (int fL = 0; fL < 6; ++fL) {
+        const FaceGrid g = face_grid(fL, m, t.xv);
+        int t1, t2;
+        tangential(kFaceDir[fL], t1, t2);
+        for (int oL = 0; oL < 8; ++oL) {
+            face_pts_order(oL, m * m, po.data());
+            for (int q = 0; q < m * m; ++q) inv[po[q]] = q;
+            for (int c2 = 0; c2 < m; ++c2)
+                for (int c1 = 0; c1 < m; ++c1) {
+                    double* o = nev.data() + (((size_t)(fL * 8 + oL) * m * m) + c2 * m + c1) * 2;
+                    if (exact) {
+                        o[0] = t.xv[c1];
+                        o[1] = t.xv[c2];
+                        continue;
+                    }
+                    // the flux point's index q on the face; the reference
+                    // evaluates the normal at the face-0 points' first two
+                    // coordinates of index q, (t_q, s_q), as (s, t) of face fL
+                    const int q = inv[g.g2c[c2 * m + c1]];
+                    const double sq = t.xv[q % m], tq = t.xv[q / m];
+                    double xi[3];
+                    face_map(fL, tq, sq, xi);
+                    o[0] = xi[t1];
+                    o[1] = xi[t2];
+                }
+        }
+    }
+    return nev;
+}
+
+}  // namespace
+
+FaceTables box_face_tables(const CaseConfig& c) {
+    FaceTables ft;
+    const KTables t = make_tables(c);
+    const int m = c.order + 2;
+    const long nx = c.nelem[0], ny = c.nelem[1], nz = c.nelem[2], E = nx * ny * nz;
+    ft.frec.resize((size_t)E * 12);
+    for (long iz = 0; iz < nz; ++iz)
+        for (long iy = 0; iy < ny; ++iy)
+            for (long ix = 0; ix < nx; ++ix) {
+                const long e = ix + nx * (iy + ny * iz);
+                for (int d = 0; d < 3; ++d)
+                    for (int side = 0; side < 2; ++side) {
+                        long cc[3] = {ix, iy, iz};
+                        cc[d] = (cc[d] + (side ? 1 : -1) + c.nelem[d]) % c.nelem[d];
+                        const long en = cc[0] + nx * (cc[1] + ny * cc[2]);
+                        const unsigned meta = (unsigned)face_id(d, 1 - side) | (0u << 3) | ((unsigned)side << 6) |
+                                              ((unsigned)(face_id(d, 1) * 8) << 8);
+                        ft.frec[((size_t)e * 6 + d * 2 + side) * 2] = (unsigned)en;
+                        ft.frec[((size_t)e * 6 + d * 2 + side) * 2 + 1] = meta;
+                    }
+            }
+    ft.nev = normal_points(t, m, c.exact_normals);
+    return ft;
+}
+
+FaceTables mesh_face_tables(const CaseConfig& c, long E, long F, const int* iface6) {
+    FaceTables ft;
+    const KTables t = make_tables(c);
+    const int m = c.order + 2;
+    std::vector<FaceGrid> grid;
+    for (int f = 0; f < 6; ++f) grid.push_back(face_grid(f, m, t.xv));
+    ft.frec.assign((size_t)E * 12, 0);
+    std::vector<char> seen((size_t)E * 6, 0);
+    std::vector<int> poL(m * m), poR(m * m), invL(m * m), invR(m * m), b_of_a(m * m), a_of_b(m * m);
+    for (long f = 0; f < F; ++f) {
+        const int* r = iface6 + f * 6;
+        const long eL = r[0], eR = r[3];
+        const int fL = r[1], oL = r[2], fR = r[4], oR = r[5];
+        if (eL < 0 || eL >= E || eR < 0 || eR >= E || fL < 0 || fL > 5 || fR < 0 || fR > 5)
+            throw InputError("IFaceData row " + std::to_string(f) + " is out of range");
+        face_pts_order(oL, m * m, poL.data());
+        face_pts_order(oR, m * m, poR.data());
+        for (int q = 0; q < m * m; ++q) {
+            invL[poL[q]] = q;
+            invR[poR[q]] = q;
+        }
+        // L grid point -> canonical L point -> face point q -> canonical R
+        // point -> R grid point (the pairing of construct_face_states,
+        // src/solver/base.cpp:287-327)
+        for (int g = 0; g < m * m; ++g) {
+            const int q = invL[grid[fL].g2c[g]];
+            b_of_a[g] = grid[fR].c2g[poR[q]];
+            const int qq = invR[grid[fR].g2c[g]];
+            a_of_b[g] = grid[fL].c2g[poL[qq]];
+        }
+        const int sL = kFaceDir[fL] * 2 + kFaceSide[fL], sR = kFaceDir[fR] * 2 + kFaceSide[fR];
+        if (seen[(size_t)eL * 6 + sL]++ || seen[(size_t)eR * 6 + sR]++)
+            throw InputError("IFaceData: an element face appears twice");
+        const unsigned lfo = (unsigned)(fL * 8 + oL) << 8;
+        ft.frec[((size_t)eL * 6 + sL) * 2] = (unsigned)eR;
+        ft.frec[((size_t)eL * 6 + sL) * 2 + 1] = (unsigned)fR | ((unsigned)map_code(b_of_a, m) << 3) | (1u << 6) | lfo;
+        ft.frec[((size_t)eR * 6 + sR) * 2] = (unsigned)eL;
+        ft.frec[((size_t)eR * 6 + sR) * 2 + 1] = (unsigned)fL | ((unsigned)map_code(a_of_b, m) << 3) | lfo;
+    }
+    for (size_t i = 0; i < seen.size(); ++i)
+        if (!seen[i])
+            throw UnsupportedError("element " + std::to_string(i / 6) +
+                                   " has a face without a neighbour: boundary faces are not on this path "
+                                   "(periodic meshes only)");
+    ft.nev = normal_points(t, m, c.exact_normals);
+    return ft;
 }
 
 KIC make_ic(const CaseConfig& c) {

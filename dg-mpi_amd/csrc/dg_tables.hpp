@@ -37,4 +37,19 @@ void box_node(const CaseConfig& c, long i, long j, long k, double* x);
 // (the reference's hex node order), element e = ix + nx (iy + ny iz).
 std::vector<double> element_nodes(const CaseConfig& c);
 
+// Face records and normal-evaluation points of the general path (the
+// layout KGeom::frec / KGeom::nev describe).
+struct FaceTables {
+    std::vector<unsigned> frec;   // [E][6][2]
+    std::vector<double> nev;      // [48][(p+2)^2][2]
+};
+
+// Records of the periodic box (structured neighbours, aligned frames).
+FaceTables box_face_tables(const CaseConfig& c);
+
+// Records of a mesh given by the reference's IFaceData 6-tuples (elemL,
+// faceL, orientL, elemR, faceR, orientR), src/mesh/mesh.cpp:140-160.  Every
+// element face must be an interior face (periodic, no boundary faces).
+FaceTables mesh_face_tables(const CaseConfig& c, long E, long F, const int* iface6);
+
 }  // namespace dg

@@ -32,7 +32,7 @@ BASIS_NAMES = ["LagrangeHex", "LegendreHex"]
 STEPPER_NAMES = ["FE", "RK4"]
 
 # every symbol include/dg_amd.h declares
-EXPORTS = ["dg_create", "dg_destroy", "dg_last_error", "dg_get_info", "dg_init_state",
+EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get_info", "dg_init_state",
            "dg_set_state", "dg_get_state", "dg_state_device_ptr", "dg_residual",
            "dg_apply_inv_mass", "dg_step", "dg_solve", "dg_l2_error", "dg_integrals",
            "dg_max_wavespeed", "dg_set_time", "dg_set_stream"]
@@ -68,6 +68,8 @@ def lib():
         vp, dp = C.c_void_p, C.POINTER(C.c_double)
         sig = {
             "dg_create": [C.c_char_p, C.POINTER(vp)],
+            "dg_create_mesh": [C.c_char_p, C.c_long, dp, C.c_long, C.POINTER(C.c_long), C.c_long,
+                               C.POINTER(C.c_int), C.POINTER(vp)],
             "dg_destroy": [vp],
             "dg_get_info": [vp, C.POINTER(Info)],
             "dg_init_state": [vp],
@@ -101,9 +103,23 @@ def _ptr(a):
 class Solver:
     """One DG case on the current HIP device."""
 
-    def __init__(self, toml_text):
+    def __init__(self, toml_text, mesh=None):
+        """mesh: None for the periodic box of the TOML case, else (node
+        coordinates [n][3], element nodes [E][8], interior faces [F][6])
+        in the layout of the reference's mesh file."""
         h = C.c_void_p()
-        rc = lib().dg_create(toml_text.encode(), C.byref(h))
+        if mesh is None:
+            rc = lib().dg_create(toml_text.encode(), C.byref(h))
+        else:
+            xyz = np.ascontiguousarray(mesh[0], dtype=np.float64)
+            e2n = np.ascontiguousarray(mesh[1], dtype=np.int64)
+            ifc = np.ascontiguousarray(mesh[2], dtype=np.int32)
+            if xyz.ndim != 2 or xyz.shape[1] != 3 or e2n.ndim != 2 or e2n.shape[1] != 8 or \
+                    ifc.ndim != 2 or ifc.shape[1] != 6:
+                raise ValueError("mesh arrays must be [n][3], [E][8] and [F][6]")
+            rc = lib().dg_create_mesh(toml_text.encode(), xyz.shape[0], _ptr(xyz), e2n.shape[0],
+                                      e2n.ctypes.data_as(C.POINTER(C.c_long)), ifc.shape[0],
+                                      ifc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h))
         if rc != 0:
             raise DGError(rc, lib().dg_last_error(None).decode())
         self._h = h
@@ -228,16 +244,19 @@ class Solver:
 def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(-5., -5., -5.),
               upper=(5., 5., 5.), stepper="RK4", timestep=None, cfl=None, end=1.0, gamma=1.4,
               gas_constant=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0),
-              warp=0.0, normals="reference"):
+              warp=0.0, normals="reference", box=True):
     """TOML text of a periodic-box case (keys of the reference input decks;
-    warp > 0 gives the non-affine box)."""
+    warp > 0 gives the non-affine box).  box=False: the case of a mesh
+    passed to Solver(..., mesh=...), whose [Mesh] holds only the normals
+    placement and the periodic cell (lower, upper) of the exact solutions."""
     st = f"timestep = {timestep!r}" if timestep is not None else f"cfl = {cfl!r}"
     arr = lambda v: "[" + ", ".join(repr(float(x)) for x in v) + "]"
-    return f"""[Mesh]
-type = "PeriodicBox"
+    mesh = (f"""type = "PeriodicBox"
 warp = {float(warp)!r}
-normals = "{normals}"
 nelem = [{nelem[0]}, {nelem[1]}, {nelem[2]}]
+""" if box else "")
+    return f"""[Mesh]
+{mesh}normals = "{normals}"
 lower = {arr(lower)}
 upper = {arr(upper)}
 

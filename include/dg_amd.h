@@ -77,6 +77,21 @@ typedef struct {
  * HIP device.  The state is left at zero; call dg_init_state. */
 int dg_create(const char* toml_text, dg_solver** out);
 
+/* The same for a hex mesh given as the arrays of the reference's mesh file
+ * (Mesh<dim>::read_mesh_file, src/mesh/mesh.cpp:79-175): node_coords
+ * [num_nodes][3], elem_nodes [num_elems][8] (node 4k + 2j + i of the
+ * element's own reference frame, Q1 hexes) and iface_data [num_ifaces][6]
+ * = (elemL, faceL, orientL, elemR, faceR, orientR) with the face ids and
+ * orientations of src/numerics/basis/shape.cpp:296-428.  Every element
+ * face must appear in exactly one interior face (periodic or closed
+ * meshes; boundary face groups are not supported).  The TOML case holds
+ * no [Mesh] keys other than normals = "reference" | "exact"; the element
+ * metric is computed at every quadrature point (non-affine elements are
+ * allowed) and the per-element inverse mass matrices are formed on the
+ * device (orders 1-4). */
+int dg_create_mesh(const char* toml_text, long num_nodes, const double* node_coords, long num_elems,
+                   const long* elem_nodes, long num_ifaces, const int* iface_data, dg_solver** out);
+
 /* Release device buffers and the handle.  Accepts NULL. */
 int dg_destroy(dg_solver* s);
 

@@ -93,3 +93,64 @@ def test_valid_case_without_gpu_fails_loudly():
     assert rc == 2 and "device" in msg.lower()
     with pytest.raises(dg_amd.DGError):
         dg_amd.Solver(BASE)
+
+
+# ---- dg_create_mesh: mesh arrays in the reference's mesh-file layout ----
+
+MESH_CASE = dg_amd.case_toml(order=2, basis="LagrangeHex", timestep=0.01, lower=(0, 0, 0), upper=(1, 1, 1),
+                             ic="Uniform", ic_data=(1.0, 0.1, 0.2, 0.3, 2.5), box=False)
+
+
+def _mesh():
+    import meshgen
+    return meshgen.box_mesh((2, 3, 2), (0, 0, 0), (1, 1, 1), warp=0.1, seed=3, rotate=True, shuffle=True,
+                            random_left=True, m=4)
+
+
+def _create_mesh(text, xyz, e2n, ifc):
+    import numpy as np
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    e2n = np.ascontiguousarray(e2n, dtype=np.int64)
+    ifc = np.ascontiguousarray(ifc, dtype=np.int32)
+    h = C.c_void_p()
+    rc = dg_amd.lib().dg_create_mesh(text.encode(), len(xyz), xyz.ctypes.data_as(C.POINTER(C.c_double)), len(e2n),
+                                     e2n.ctypes.data_as(C.POINTER(C.c_long)), len(ifc),
+                                     ifc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h))
+    msg = dg_amd.lib().dg_last_error(None).decode()
+    if rc == 0:
+        dg_amd.lib().dg_destroy(h)
+    return rc, msg
+
+
+def test_mesh_input_is_checked_before_the_device():
+    """A valid mesh gets as far as the device (status 2 without a GPU);
+    broken IFaceData, node indices and keys are input errors on any host
+    (Mesh<dim>::read_mesh_file's checks, src/mesh/mesh.cpp:79-175)."""
+    from dg_amd import hip
+    xyz, e2n, ifc = _mesh()
+    rc, msg = _create_mesh(MESH_CASE, xyz, e2n, ifc)
+    assert rc == (0 if hip.device_count() > 0 else 2), msg
+    bad = ifc.copy()
+    bad[3, 2] = 9                                   # orientation outside 0..7
+    rc, msg = _create_mesh(MESH_CASE, xyz, e2n, bad)
+    assert rc == 1 and "orientation 9" in msg, msg
+    bad = ifc.copy()
+    bad[0, 3] = len(e2n)                            # element out of range
+    rc, msg = _create_mesh(MESH_CASE, xyz, e2n, bad)
+    assert rc == 1 and "out of range" in msg, msg
+    rc, msg = _create_mesh(MESH_CASE, xyz, e2n, ifc[:-1])   # a face without a neighbour
+    assert rc == 4 and "boundary faces" in msg, msg
+    bad = ifc.copy()
+    bad[1] = bad[0]                                 # a face used twice
+    rc, msg = _create_mesh(MESH_CASE, xyz, e2n, bad)
+    assert rc == 1 and "twice" in msg, msg
+    e = e2n.copy()
+    e[2, 5] = len(xyz)
+    rc, msg = _create_mesh(MESH_CASE, xyz, e, ifc)
+    assert rc == 1 and "node index" in msg, msg
+    rc, msg = _create_mesh(MESH_CASE.replace("order = 2", "order = 5"), xyz, e2n, ifc)
+    assert rc == 4 and "order > 4" in msg, msg
+    rc, msg = _create_mesh(MESH_CASE.replace("[Mesh]", "[Mesh]\nwarp = 0.1"), xyz, e2n, ifc)
+    assert rc == 1 and "periodic box only" in msg, msg
+    rc, msg = _create_mesh(MESH_CASE.replace("timestep = 0.01", "cfl = 0.1"), xyz, e2n, ifc)
+    assert rc == 1 and "timestep is required" in msg, msg
