@@ -221,6 +221,14 @@ int pick(const std::string& str, const std::vector<std::string>& valid) {
 
 }  // namespace
 
+std::string case_mesh_file(const std::string& toml_text) {
+    TomlDoc d = parse_toml(toml_text);
+    const TomlValue* v = find(d, "Mesh", "file");
+    if (!v) return "";
+    if (v->kind != TomlValue::STRING) throw InputError("Mesh.file must be a string");
+    return v->str;
+}
+
 CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     TomlDoc d = parse_toml(toml_text);
     CaseConfig c;

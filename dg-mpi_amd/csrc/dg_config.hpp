@@ -73,4 +73,7 @@ struct CaseConfig {
 // mesh_input: the mesh is given with the call (no box keys required)
 CaseConfig parse_case(const std::string& toml_text, bool mesh_input = false);
 
+// Mesh.file of a case ("" when the case has none: a PeriodicBox case).
+std::string case_mesh_file(const std::string& toml_text);
+
 }  // namespace dg

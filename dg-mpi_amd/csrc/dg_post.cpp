@@ -78,10 +78,18 @@ int main(int argc, char** argv) {
     std::stringstream ss;
     ss << f.rdbuf();
     dg_solver* s = nullptr;
+#ifdef DG_WITH_HDF5
+    // Mesh.file cases read their mesh file; PeriodicBox cases go to dg_create
+    if (dg_create_from_mesh_file(ss.str().c_str(), nullptr, &s) != DG_OK) {
+        std::fprintf(stderr, "dg_create: %s\n", dg_h5_last_error());
+        return 1;
+    }
+#else
     if (dg_create(ss.str().c_str(), &s) != DG_OK) {
         std::fprintf(stderr, "dg_create: %s\n", dg_last_error(nullptr));
         return 1;
     }
+#endif
     int rc = 0;
 #ifdef DG_WITH_HDF5
     if (!ends_with(data, ".dgs")) {

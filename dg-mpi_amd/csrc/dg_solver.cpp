@@ -37,6 +37,8 @@ struct dg_solver {
     unsigned* frec = nullptr;          // general path: face records [E][6][2]
     double* nev = nullptr;             // general path: normal evaluation points
     dg::KTables* dtab = nullptr;       // device copy of tab
+    std::vector<double> mesh_xyz;      // given mesh: node coordinates [n][3]
+    std::vector<long> mesh_e2n;        // given mesh: element nodes [E][8]
     std::string err;
 };
 
@@ -224,6 +226,10 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         }
         s = new dg_solver();
         s->cfg = cfg;
+        if (mesh) {
+            s->mesh_xyz.assign(mesh->coords, mesh->coords + mesh->nnodes * 3);
+            s->mesh_e2n.assign(mesh->e2n, mesh->e2n + mesh->E * 8);
+        }
         s->P = cfg.order;
         s->tab = dg::make_tables(cfg);
         s->geo = dg::make_geometry(cfg);
@@ -337,6 +343,37 @@ int dg_get_info(const dg_solver* s, dg_info* info) {
         info->flux = (int)c.flux;
         info->basis = (int)c.basis;
         info->ic = (int)c.ic;
+    });
+}
+
+int dg_get_mesh(const dg_solver* s, long* num_nodes, double* node_coords, long* elem_nodes) {
+    return guarded(const_cast<dg_solver*>(s), [&] {
+        need(s, "solver");
+        need(num_nodes, "num_nodes");
+        const auto& c = s->cfg;
+        if (!s->mesh_e2n.empty()) {
+            *num_nodes = (long)(s->mesh_xyz.size() / 3);
+            if (node_coords) std::memcpy(node_coords, s->mesh_xyz.data(), s->mesh_xyz.size() * sizeof(double));
+            if (elem_nodes) std::memcpy(elem_nodes, s->mesh_e2n.data(), s->mesh_e2n.size() * sizeof(long));
+            return;
+        }
+        // the box: (n+1)^3 grid nodes i + (nx+1)(j + (ny+1) k), the warp
+        // displacement included; element e = ix + nx (iy + ny iz)
+        const long nx = c.nelem[0], ny = c.nelem[1], nz = c.nelem[2];
+        const long Nx = nx + 1, Ny = ny + 1;
+        *num_nodes = Nx * Ny * (nz + 1);
+        if (node_coords)
+            for (long k = 0; k <= nz; ++k)
+                for (long j = 0; j <= ny; ++j)
+                    for (long i = 0; i <= nx; ++i) dg::box_node(c, i, j, k, node_coords + 3 * (i + Nx * (j + Ny * k)));
+        if (elem_nodes)
+            for (long k = 0; k < nz; ++k)
+                for (long j = 0; j < ny; ++j)
+                    for (long i = 0; i < nx; ++i) {
+                        long* en = elem_nodes + 8 * (i + nx * (j + ny * k));
+                        for (int n = 0; n < 8; ++n)
+                            en[n] = (i + (n & 1)) + Nx * ((j + ((n >> 1) & 1)) + Ny * (k + (n >> 2)));
+                    }
     });
 }
 

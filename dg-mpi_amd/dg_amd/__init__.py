@@ -32,7 +32,7 @@ BASIS_NAMES = ["LagrangeHex", "LegendreHex"]
 STEPPER_NAMES = ["FE", "RK4"]
 
 # every symbol include/dg_amd.h declares
-EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get_info", "dg_init_state",
+EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get_info", "dg_get_mesh", "dg_init_state",
            "dg_set_state", "dg_get_state", "dg_state_device_ptr", "dg_residual",
            "dg_apply_inv_mass", "dg_step", "dg_solve", "dg_l2_error", "dg_integrals",
            "dg_max_wavespeed", "dg_set_time", "dg_set_stream"]
@@ -72,6 +72,7 @@ def lib():
                                C.POINTER(C.c_int), C.POINTER(vp)],
             "dg_destroy": [vp],
             "dg_get_info": [vp, C.POINTER(Info)],
+            "dg_get_mesh": [vp, C.POINTER(C.c_long), dp, C.POINTER(C.c_long)],
             "dg_init_state": [vp],
             "dg_set_state": [vp, dp, C.c_int],
             "dg_get_state": [vp, dp, C.c_int],
@@ -125,6 +126,14 @@ class Solver:
         self._h = h
         self.info = self._info()
 
+    @classmethod
+    def _adopt(cls, h):
+        """Wrap a handle created elsewhere (libdg_amd_h5)."""
+        self = cls.__new__(cls)
+        self._h = h
+        self.info = self._info()
+        return self
+
     # -- plumbing --
     def _check(self, rc):
         if rc != 0:
@@ -168,6 +177,15 @@ class Solver:
     @property
     def num_iterations(self):
         return self._info().num_iterations
+
+    def mesh(self):
+        """(node coordinates [n][3], element nodes [E][8]) of the case."""
+        n = C.c_long()
+        self._check(lib().dg_get_mesh(self._h, C.byref(n), None, None))
+        xyz = np.zeros((n.value, 3))
+        e2n = np.zeros((self.info.num_elems, 8), dtype=np.int64)
+        self._check(lib().dg_get_mesh(self._h, C.byref(n), _ptr(xyz), e2n.ctypes.data_as(C.POINTER(C.c_long))))
+        return xyz, e2n
 
     # -- reference surface --
     def init_state_from_fcn(self):

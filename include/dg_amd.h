@@ -102,6 +102,15 @@ const char* dg_last_error(const dg_solver* s);
 /* Sizes and parameters of the case. */
 int dg_get_info(const dg_solver* s, dg_info* info);
 
+/* The mesh of the case: node coordinates [num_nodes][3] and element nodes
+ * [num_elems][8] (node 4k + 2j + i of each element's frame), as given to
+ * dg_create_mesh, or the (nx+1)(ny+1)(nz+1) grid of the box (warp included,
+ * node i + (nx+1)(j + (ny+1)k), element ix + nx(iy + ny iz)) -- the arrays
+ * the reference's Writer stores (src/io/writer.cpp:68-98).  Call with NULL
+ * arrays to get num_nodes; then node_coords holds num_nodes*3 doubles and
+ * elem_nodes num_elems*8 longs. */
+int dg_get_mesh(const dg_solver* s, long* num_nodes, double* node_coords, long* elem_nodes);
+
 /* L2 projection of the initial condition on (2p+2)^3 Gauss points, time
  * set to 0 (Solver::init_state_from_fcn, src/solver/base.cpp:113-223;
  * SolverTools::L2_projection, src/solver/tools.cpp:66-122). */

@@ -1,5 +1,5 @@
-/* Solution files in the reference's data.h5 layout, for the periodic-box
- * DG path of libdg_amd.
+/* Solution files in the reference's data.h5 layout and hex mesh files in
+ * the layout of the reference's mesh reader, for the DG path of libdg_amd.
  *
  * Implemented by libdg_amd_h5.so (host code over the HDF5 C library; no GPU
  * code).  The file layout is the one Writer::write produces
@@ -56,7 +56,42 @@ int dg_h5_read_header(const char* path, dg_h5_header* header);
  * `capacity` is the number of doubles Uc holds. */
 int dg_h5_read_coefficients(const char* path, double* Uc, long capacity, int layout);
 
-/* Write the solver's state and time to `path` (src/exec/main.cpp:106-112). */
+/* Sizes of a mesh file (Mesh::read_mesh, src/mesh/mesh.cpp:79-107). */
+typedef struct dg_h5_mesh_info {
+    long dim;                  /* "Dimension" */
+    long qorder;               /* "QOrder" (geometric order) */
+    long num_elems;            /* "nElem" */
+    long num_nodes;            /* "nNode" */
+    long num_ifaces;           /* "nIFace" */
+    long num_nodes_per_elem;   /* "nNodePerElem" */
+    long num_nodes_per_face;   /* "nNodePerFace" (1 when absent, as the reference takes it) */
+    long num_bface_groups;     /* "nBFaceGroup" (0 when absent) */
+} dg_h5_mesh_info;
+
+int dg_h5_read_mesh_info(const char* path, dg_h5_mesh_info* info);
+
+/* NodeCoords [num_nodes][dim], Elem2Nodes [num_elems][num_nodes_per_elem]
+ * and IFaceData [num_ifaces][6] of a mesh file (src/mesh/mesh.cpp:120-160),
+ * any dimension; the capacities are the lengths of the three buffers. */
+int dg_h5_read_mesh(const char* path, double* node_coords, long coords_capacity, long* elem_nodes,
+                    long elem_nodes_capacity, int* iface_data, long iface_capacity);
+
+/* Write a 3D mesh file of trilinear hexahedra (QOrder 1, 8 nodes per
+ * element, 4 per face, no boundary face groups): the datasets the
+ * reference's mesh reader opens, with the arrays of dg_create_mesh. */
+int dg_h5_write_mesh(const char* path, long num_nodes, const double* node_coords, long num_elems,
+                     const long* elem_nodes, long num_ifaces, const int* iface_data);
+
+/* dg_create_mesh with the arrays of a mesh file (the Mesh constructor,
+ * src/mesh/mesh.cpp:28-60, and Mesh::read_mesh, 79-200).  mesh_file NULL:
+ * the case's Mesh.file, a path relative to the working directory as in the
+ * reference; a case without Mesh.file (a PeriodicBox case) goes to
+ * dg_create.  The file must hold a 3D, QOrder 1 hex mesh without boundary
+ * face groups.  On failure dg_h5_last_error() holds the message. */
+int dg_create_from_mesh_file(const char* toml_text, const char* mesh_file, dg_solver** out);
+
+/* Write the solver's state and time to `path` (src/exec/main.cpp:106-112),
+ * with the solver's mesh (dg_get_mesh). */
 int dg_write_solution(const dg_solver* s, const char* path);
 
 /* Load the coefficients of `path` into the solver's state
