@@ -1009,6 +1009,8 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
     team_for<MM * V, TS>(lane, [&](int t) {
         const int s = t % V, ba = t / V, a = ba % M, b = ba / M;
         const double wab = LT.w[a] * LT.w[b];
+        // pressure enters the momentum component s of flux direction s - 1
+        const double e1 = s == 1 ? 1. : 0., e2 = s == 2 ? 1. : 0., e3 = s == 3 ? 1. : 0.;
         double xx[N], yy[N], zz[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) xx[k] = yy[k] = zz[k] = 0.;
@@ -1019,18 +1021,19 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
             if constexpr (MAPPED) {
                 const double* q = Qd + (c * MM + ba) * 17;
                 const double Ws = q[s];
-                const double px = (s >= 1 && s <= 3) ? q[8 + (s - 1)] : 0.;
-                const double py = (s >= 1 && s <= 3) ? q[11 + (s - 1)] : 0.;
-                const double pz = (s >= 1 && s <= 3) ? q[14 + (s - 1)] : 0.;
-                gx = (Ws * q[5] + px) * wq;
-                gy = (Ws * q[6] + py) * wq;
-                gz = (Ws * q[7] + pz) * wq;
+                // p C_d . e_s for the momentum components
+                const double px = fma(e1, q[8], fma(e2, q[9], e3 * q[10]));
+                const double py = fma(e1, q[11], fma(e2, q[12], e3 * q[13]));
+                const double pz = fma(e1, q[14], fma(e2, q[15], e3 * q[16]));
+                gx = fma(Ws, q[5], px) * wq;
+                gy = fma(Ws, q[6], py) * wq;
+                gz = fma(Ws, q[7], pz) * wq;
             } else {
                 const double* q = Qd + (c * MM + ba) * 9;
                 const double Ws = q[s], p = q[8];
-                gx = (Ws * q[5] + (s == 1 ? p : 0.)) * (G.gvol[0] * wq);
-                gy = (Ws * q[6] + (s == 2 ? p : 0.)) * (G.gvol[1] * wq);
-                gz = (Ws * q[7] + (s == 3 ? p : 0.)) * (G.gvol[2] * wq);
+                gx = fma(Ws, q[5], e1 * p) * (G.gvol[0] * wq);
+                gy = fma(Ws, q[6], e2 * p) * (G.gvol[1] * wq);
+                gz = fma(Ws, q[7], e3 * p) * (G.gvol[2] * wq);
             }
 #pragma unroll
             for (int k = 0; k < N; ++k) {
@@ -1193,44 +1196,70 @@ struct StageLoads {
             own[r] = t < NB ? src[t] : 0.;
         }
     }
-    __device__ __forceinline__ void load_neighbours(const double* Ust, unsigned e, const KGeom& G, int lane) {
-        int ec[3];
-        elem_ijk(e, G, ec);
+    // one neighbour line per task t = ((ds N + t2) N + t1) V + s; the six
+    // neighbour elements are wave-uniform and picked per lane by slot ds
+    // (selects, no per-lane wrap arithmetic), the line is addressed by a
+    // base and the normal stride of the neighbour's face direction
+    template <class Pick>
+    __device__ __forceinline__ void load_lines(const double* Ust, Pick pick, int lane) {
 #pragma unroll
         for (int r = 0; r < RN; ++r) {
             const int t = r * TS + lane;
             if (t < NT) {
                 const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
-                const int d = ds >> 1, side = ds & 1;
-                const int off = side ? 1 : -1;
-                const unsigned en = elem_at(ec[0] + (d == 0 ? off : 0), ec[1] + (d == 1 ? off : 0),
-                                            ec[2] + (d == 2 ? off : 0), G);
-                const double* Un = Ust + (size_t)en * NB;
+                unsigned en;
+                int dn;
+                pick(ds, en, dn);
+                const int sr = dn == 0 ? V : (dn == 1 ? N * V : NN * V);
+                const int s1 = dn == 0 ? N * V : V;
+                const int s2 = dn == 2 ? N * V : NN * V;
+                const double* ln = Ust + (size_t)en * NB + (t1 * s1 + t2 * s2 + s);
 #pragma unroll
-                for (int q = 0; q < N; ++q) nl[r][q] = Un[bidx_d<N>(d, q, t1, t2) * V + s];
+                for (int q = 0; q < N; ++q) nl[r][q] = ln[q * sr];
             }
         }
     }
+    __device__ __forceinline__ void load_neighbours(const double* Ust, unsigned e, const KGeom& G, int lane) {
+        int ec[3];
+        elem_ijk(e, G, ec);
+        const unsigned n0 = elem_at(ec[0] - 1, ec[1], ec[2], G), n1 = elem_at(ec[0] + 1, ec[1], ec[2], G);
+        const unsigned n2 = elem_at(ec[0], ec[1] - 1, ec[2], G), n3 = elem_at(ec[0], ec[1] + 1, ec[2], G);
+        const unsigned n4 = elem_at(ec[0], ec[1], ec[2] - 1, G), n5 = elem_at(ec[0], ec[1], ec[2] + 1, G);
+        load_lines(Ust, [&](int ds, unsigned& en, int& dn) {
+            en = n0;
+            en = ds == 1 ? n1 : en;
+            en = ds == 2 ? n2 : en;
+            en = ds == 3 ? n3 : en;
+            en = ds == 4 ? n4 : en;
+            en = ds == 5 ? n5 : en;
+            dn = ds >> 1;
+        }, lane);
+    }
     // general connectivity: the neighbour and its face come from the face
-    // records; the lines run along the neighbour's own normal direction and
-    // the slab is kept in the neighbour's tangential frame
+    // records (read once per element, wave-uniform); the lines run along the
+    // neighbour's own normal direction and the slab is kept in the
+    // neighbour's tangential frame
     __device__ __forceinline__ void load_neighbours_gen(const double* Ust, unsigned e, const KGeom& G,
                                                         int lane) {
+        const unsigned* rec = G.frec + (size_t)e * 12;
+        const unsigned n0 = rec[0], n1 = rec[2], n2 = rec[4], n3 = rec[6], n4 = rec[8], n5 = rec[10];
+        // neighbour face ids, 3 bits each
+        const unsigned fpack = (rec[1] & 7) | ((rec[3] & 7) << 3) | ((rec[5] & 7) << 6) | ((rec[7] & 7) << 9) |
+                               ((rec[9] & 7) << 12) | ((rec[11] & 7) << 15);
+        load_lines(Ust, [&](int ds, unsigned& en, int& dn) {
+            en = n0;
+            en = ds == 1 ? n1 : en;
+            en = ds == 2 ? n2 : en;
+            en = ds == 3 ? n3 : en;
+            en = ds == 4 ? n4 : en;
+            en = ds == 5 ? n5 : en;
+            dn = face_dir((fpack >> (3 * ds)) & 7);
+        }, lane);
 #pragma unroll
         for (int r = 0; r < RN; ++r) {
             const int t = r * TS + lane;
-            nside[r] = 0;
-            if (t < NT) {
-                const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
-                const unsigned* rec = G.frec + ((size_t)e * 6 + ds) * 2;
-                const unsigned en = rec[0];
-                const int fn = rec[1] & 7;
-                const int dn = face_dir(fn);
-                nside[r] = face_side(fn);
-                const double* Un = Ust + (size_t)en * NB;
-#pragma unroll
-                for (int q = 0; q < N; ++q) nl[r][q] = Un[bidx_d<N>(dn, q, t1, t2) * V + s];
-            }
+            const int ds = t / (V * NN);
+            nside[r] = ds < 6 ? face_side((fpack >> (3 * ds)) & 7) : 0;
         }
     }
     __device__ __forceinline__ void load_rk(const StageArgs& A, unsigned e, int lane) {
