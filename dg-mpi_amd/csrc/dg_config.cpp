@@ -239,8 +239,8 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     int pt = pick(phys, {"Euler", "NavierStokes"});          // enums.h:56-66
     if (pt != 0) throw UnsupportedError("Physics.name = NavierStokes is not implemented by this path");
     int dim = get_int(d, "Physics", "dim", true, 3);
-    if (dim == 2) throw UnsupportedError("Physics.dim = 2 (quadrilateral path) is not implemented by this path");
-    if (dim != 3) throw InputError("Physics.dim must be 2 or 3");
+    if (dim != 2 && dim != 3) throw InputError("Physics.dim must be 2 or 3");
+    c.dim = dim;
     c.gamma = get_num(d, "Physics", "gamma", false, 1.4);
     c.gas_constant = get_num(d, "Physics", "GasConstant", false, 287.05);
     std::string nf = get_str(d, "Physics", "convective_flux_fcn", false, "LaxFriedrichs");
@@ -255,10 +255,13 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
         "LagrangeGLL1D", "LagrangeGLL2D", "LagrangeGLL3D", "LagrangeGL2D", "LagrangeGL3D",
         "TriangleLagrange"};                                     // enums.h:109-139
     int bi = pick(bname, bases);
-    if (bi == 2) c.basis = Basis::LagrangeHex;
-    else if (bi == 5) c.basis = Basis::LegendreHex;
+    // the tensor bases share their 1D functions; the shape follows Physics.dim
+    if (bi == 2 || bi == 1) c.basis = Basis::LagrangeHex;
+    else if (bi == 5 || bi == 4) c.basis = Basis::LegendreHex;
     else throw UnsupportedError("Numerics.basis = " + bname + " is not implemented by this path "
-                                "(LagrangeHex, LegendreHex)");
+                                "(LagrangeHex, LegendreHex; LagrangeQuad, LegendreQuad with dim = 2)");
+    if ((bi == 1 || bi == 4) != (dim == 2))
+        throw InputError("Numerics.basis = " + bname + " does not match Physics.dim = " + std::to_string(dim));
     std::string quad = get_str(d, "Numerics", "quad", false, "GaussLegendre");
     int qi = pick(quad, {"GaussLegendre", "GaussLobatto", "TriangleDunavant", "TriangleGaussLegendre"});
     if (qi != 0) throw UnsupportedError("Numerics.quad = " + quad + " is not implemented");
@@ -281,16 +284,22 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     {
         int idx = pick(icn, {"Uniform", "Gaussian", "Sphere", "IsentropicVortex", "ExtrudedIsentropicVortex",
                              "GaussianBump", "TaylorGreen"});
-        static const IC map[] = {IC::Uniform, IC::Uniform, IC::Sphere, IC::IsentropicVortex,
+        static const IC map[] = {IC::Uniform, IC::Gaussian, IC::Sphere, IC::IsentropicVortex,
                                  IC::ExtrudedIsentropicVortex, IC::GaussianBump, IC::TaylorGreen};
-        if (idx == 1) throw UnsupportedError("InitialCondition.name = Gaussian is a 2D initial condition");
         c.ic = map[idx];
+        // Physics<2>::call_IC and Physics<3>::call_IC (physics/base/base.cpp:77-103)
+        if (dim == 2 && !(c.ic == IC::Uniform || c.ic == IC::Gaussian || c.ic == IC::IsentropicVortex))
+            throw UnsupportedError("InitialCondition.name = " + icn + " is a 3D initial condition");
+        if (dim == 3 && c.ic == IC::Gaussian)
+            throw UnsupportedError("InitialCondition.name = Gaussian is a 2D initial condition");
     }
     c.ic_data = get_num_array(d, "InitialCondition", "data", false);
     if (c.ic_data.size() > 10) throw InputError("InitialCondition.data holds at most 10 values");
     size_t need = 0;
     switch (c.ic) {
-    case IC::Uniform: need = 5; break;
+    case IC::Uniform: need = dim == 2 ? 4 : 5; break;
+    case IC::Gaussian: need = 4; break;
+    case IC::IsentropicVortex: need = dim == 2 ? 5 : 0; break;
     case IC::ExtrudedIsentropicVortex: need = 5; break;
     case IC::GaussianBump: need = 9; break;
     case IC::TaylorGreen: need = 1; break;
@@ -298,11 +307,14 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     }
     if (c.ic_data.size() < need)
         throw InputError("InitialCondition.data needs " + std::to_string(need) + " values for " + icn);
-    if (c.ic == IC::ExtrudedIsentropicVortex && c.gas_constant != 1.0)
+    if (((dim == 2 && c.ic == IC::IsentropicVortex) || c.ic == IC::ExtrudedIsentropicVortex) && c.gas_constant != 1.0)
         throw InputError("ExtrudedIsentropicVortex requires Physics.GasConstant = 1 "
                          "(src/physics/euler/functions.cpp:107)");
 
     // [Mesh]
+    if (mesh_input && dim == 2)
+        throw UnsupportedError("quadrilateral meshes from arrays or files are not on this path "
+                               "(Physics.dim = 2 takes Mesh.type = \"PeriodicBox\")");
     if (mesh_input) {
         // the mesh comes with the call (dg_create_mesh); lower / upper, if
         // given, are the periodic cell of the advected exact solutions
@@ -332,21 +344,29 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
         missing("Mesh", "type");
     }
     if (mtype != "PeriodicBox") throw InputError("Mesh.type must be \"PeriodicBox\"");
+    // dim entries each (2D: the box is nelem[0] x nelem[1], nelem[2] = 1)
+    const size_t nd = (size_t)dim;
+    const std::string ndw = dim == 2 ? "2" : "3";
     std::vector<double> ne = get_num_array(d, "Mesh", "nelem", true);
-    if (ne.size() != 3) throw InputError("Mesh.nelem must hold 3 integers");
-    for (int i = 0; i < 3; ++i) {
+    if (ne.size() != nd) throw InputError("Mesh.nelem must hold " + ndw + " integers");
+    for (size_t i = 0; i < nd; ++i) {
         if (ne[i] < 1 || ne[i] != std::floor(ne[i])) throw InputError("Mesh.nelem entries must be positive integers");
         c.nelem[i] = (int)ne[i];
     }
     std::vector<double> lo = get_num_array(d, "Mesh", "lower", false);
     std::vector<double> hi = get_num_array(d, "Mesh", "upper", false);
     if (!lo.empty()) {
-        if (lo.size() != 3) throw InputError("Mesh.lower must hold 3 numbers");
-        for (int i = 0; i < 3; ++i) c.lower[i] = lo[i];
+        if (lo.size() != nd) throw InputError("Mesh.lower must hold " + ndw + " numbers");
+        for (size_t i = 0; i < nd; ++i) c.lower[i] = lo[i];
     }
     if (!hi.empty()) {
-        if (hi.size() != 3) throw InputError("Mesh.upper must hold 3 numbers");
-        for (int i = 0; i < 3; ++i) c.upper[i] = hi[i];
+        if (hi.size() != nd) throw InputError("Mesh.upper must hold " + ndw + " numbers");
+        for (size_t i = 0; i < nd; ++i) c.upper[i] = hi[i];
+    }
+    if (dim == 2) {
+        c.nelem[2] = 1;
+        c.lower[2] = 0.0;
+        c.upper[2] = 1.0;
     }
     for (int i = 0; i < 3; ++i)
         if (!(c.upper[i] > c.lower[i])) throw InputError("Mesh.upper must exceed Mesh.lower");
@@ -355,6 +375,7 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     c.warp = get_num(d, "Mesh", "warp", false, 0.0);
     if (!(c.warp >= 0.0 && c.warp <= 0.25))
         throw InputError("Mesh.warp must lie in [0, 0.25] (node displacement in element widths)");
+    if (dim == 2 && c.warp > 0.0) throw UnsupportedError("Mesh.warp with Physics.dim = 2 (affine quadrilaterals only)");
     const std::string nrm = get_str(d, "Mesh", "normals", false, "reference");
     if (nrm == "exact") c.exact_normals = true;
     else if (nrm != "reference") throw InputError("Mesh.normals must be \"reference\" or \"exact\"");

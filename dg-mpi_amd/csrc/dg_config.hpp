@@ -41,9 +41,12 @@ enum class Basis { LagrangeHex = 0, LegendreHex = 1 };
 enum class Flux { LaxFriedrichs = 0, HLLC = 1, Roe = 2 };
 enum class Stepper { FE = 0, RK4 = 1 };
 enum class IC { Uniform = 0, Sphere = 1, IsentropicVortex = 2, ExtrudedIsentropicVortex = 3,
-                GaussianBump = 4, TaylorGreen = 5 };
+                GaussianBump = 4, TaylorGreen = 5, Gaussian = 6 };
 
 struct CaseConfig {
+    // [Physics] dim: 3 (hexahedra, 5 state variables) or 2 (quadrilaterals,
+    // 4 state variables; the box has nelem[2] = 1)
+    int dim = 3;
     // [Numerics] (src/numerics/numerics_data.cpp:229-300)
     int order = 0;
     Basis basis = Basis::LagrangeHex;

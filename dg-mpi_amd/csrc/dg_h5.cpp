@@ -173,15 +173,17 @@ ParallelInfo parallel_info(hid_t file, const std::string& name) {
 
 std::string errstr_path(const char* path) { return path ? std::string(path) : std::string("(null)"); }
 
-// data.h5 of a hex mesh (Writer::write, src/io/writer.cpp:10-65, one rank)
-void write_solution_file(const char* path, long nn, const double* xyz, long E, const long* e2n, int num_basis,
-                         int num_state_vars, double time, const double* Uc, int layout) {
+// data.h5 of a hex (dim 3) or quadrilateral (dim 2) mesh (Writer::write,
+// src/io/writer.cpp:10-65, one rank): xyz [nn][dim], e2n [E][2^dim]
+void write_solution_file(const char* path, int dim, long nn, const double* xyz, long E, const long* e2n,
+                         int num_basis, int num_state_vars, double time, const double* Uc, int layout) {
     if (layout != DG_LAYOUT_RIGHT && layout != DG_LAYOUT_LEFT)
         throw ArgError("layout must be DG_LAYOUT_RIGHT or DG_LAYOUT_LEFT");
     if (num_basis < 1 || num_state_vars < 1 || E < 1 || nn < 1) throw ArgError("bad sizes");
     if (E > 0xffffffffL || nn > 0xffffffffL) throw ArgError("mesh too large for 32-bit IDs");
     quiet_hdf5();
-    std::vector<unsigned> nid((size_t)nn), eid((size_t)E), en((size_t)E * 8);
+    const unsigned npe = dim == 2 ? 4u : 8u;
+    std::vector<unsigned> nid((size_t)nn), eid((size_t)E), en((size_t)E * npe);
     for (long n = 0; n < nn; ++n) nid[(size_t)n] = (unsigned)n;
     for (long e = 0; e < E; ++e) eid[(size_t)e] = (unsigned)e;
     for (size_t i = 0; i < en.size(); ++i) {
@@ -190,19 +192,19 @@ void write_solution_file(const char* path, long nn, const double* xyz, long E, c
     }
     Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose, "create " + errstr_path(path));
     write_attr(file, "Number of Ranks", H5T_NATIVE_UINT, 1u);
-    write_attr(file, "Number of Dimensions", H5T_NATIVE_UINT, 3u);
+    write_attr(file, "Number of Dimensions", H5T_NATIVE_UINT, (unsigned)dim);
     write_attr(file, "Number of Elements", H5T_NATIVE_UINT, (unsigned)E);
     write_attr(file, "Number of Nodes", H5T_NATIVE_UINT, (unsigned)nn);
-    write_attr(file, "Number of Nodes Per Element", H5T_NATIVE_UINT, 8u);
+    write_attr(file, "Number of Nodes Per Element", H5T_NATIVE_UINT, npe);
     write_attr(file, "Number of Basis Functions", H5T_NATIVE_UINT, (unsigned)num_basis);
     write_attr(file, "Number of State Variables", H5T_NATIVE_UINT, (unsigned)num_state_vars);
     write_attr(file, "Solver Final Time", H5T_NATIVE_DOUBLE, time);
     const unsigned epart = (unsigned)E;
     write_1d(file, "Number of Elements per Partition", H5T_NATIVE_UINT, 1, &epart);
     write_attr(file, "Stored Layout", H5T_NATIVE_INT, layout == DG_LAYOUT_RIGHT ? 1 : 0);
-    write_parallel(file, "Node Coordinates", H5T_NATIVE_DOUBLE, {(hsize_t)nn, 3}, xyz);
+    write_parallel(file, "Node Coordinates", H5T_NATIVE_DOUBLE, {(hsize_t)nn, (hsize_t)dim}, xyz);
     write_parallel(file, "Local to Global Node ID", H5T_NATIVE_UINT, {(hsize_t)nn}, nid.data());
-    write_parallel(file, "Element Global Node IDs", H5T_NATIVE_UINT, {(hsize_t)E, 8}, en.data());
+    write_parallel(file, "Element Global Node IDs", H5T_NATIVE_UINT, {(hsize_t)E, (hsize_t)npe}, en.data());
     write_parallel(file, "Local to Global Element IDs", H5T_NATIVE_UINT, {(hsize_t)E}, eid.data());
     write_parallel(file, "Solution Coefficients", H5T_NATIVE_DOUBLE,
                    {(hsize_t)E, (hsize_t)num_basis, (hsize_t)num_state_vars}, Uc);
@@ -304,7 +306,7 @@ int dg_h5_write_box(const char* path, const int nelem[3], const double lower[3],
                     for (int n = 0; n < 8; ++n)
                         e2n[(size_t)e * 8 + n] = (i + (n & 1)) + Nx * ((j + ((n >> 1) & 1)) + Ny * (k + (n >> 2)));
                 }
-        write_solution_file(path, nn, xyz.data(), E, e2n.data(), num_basis, num_state_vars, time, Uc, layout);
+        write_solution_file(path, 3, nn, xyz.data(), E, e2n.data(), num_basis, num_state_vars, time, Uc, layout);
     });
 }
 
@@ -487,10 +489,10 @@ int dg_write_solution(const dg_solver* s, const char* path) {
     }
     return guarded([&] {
         if (!path) throw ArgError("dg_write_solution: NULL path");
-        std::vector<double> xyz((size_t)nn * 3);
-        std::vector<long> e2n((size_t)info.num_elems * 8);
+        std::vector<double> xyz((size_t)nn * info.dim);
+        std::vector<long> e2n((size_t)info.num_elems * (info.dim == 2 ? 4 : 8));
         if (dg_get_mesh(s, &nn, xyz.data(), e2n.data()) != DG_OK) throw std::runtime_error(dg_last_error(s));
-        write_solution_file(path, nn, xyz.data(), info.num_elems, e2n.data(), info.num_basis,
+        write_solution_file(path, info.dim, nn, xyz.data(), info.num_elems, e2n.data(), info.num_basis,
                             info.num_state_vars, info.time, U.data(), DG_LAYOUT_RIGHT);
     });
 }

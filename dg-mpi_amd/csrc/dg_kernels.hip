@@ -1797,6 +1797,8 @@ __global__ __launch_bounds__(256) void mass_kernel(double* imm, double* jmin, co
     team_for<NB * NB, TS>(lane, [&](int t) { imm[(size_t)e * NB * NB + t] = A[t]; });
 }
 
+#include "dg_kernels_2d.inc"
+
 // ---------------------------------------------------------------------------
 // Dispatch
 // ---------------------------------------------------------------------------
@@ -1838,30 +1840,50 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
 
 hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, const KGeom& g,
                         hipStream_t st) {
+    if (g.dim == 2) {
+        DG_DISPATCH_P(P, return stage2d_p<PP>(flux, a, t, g, st));
+    }
     DG_DISPATCH_P(P, return stage_p<PP>(flux, a, t, g, st));
     return hipSuccess;
 }
 
 hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,
                            const KGeom& g, hipStream_t st) {
+    if (g.dim == 2) {
+        DG_DISPATCH_P(P, hipLaunchKernelGGL((inv_mass2d_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, dt, R, dU, t, g));
+        return hipGetLastError();
+    }
     DG_DISPATCH_P(P, hipLaunchKernelGGL((inv_mass_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, dt, R, dU, t, g));
     return hipGetLastError();
 }
 
 hipError_t launch_init_state(int P, double time, double* U, const KTables* t, const KGeom& g,
                              const KIC& ic, hipStream_t st) {
+    if (g.dim == 2) {
+        DG_DISPATCH_P(P, hipLaunchKernelGGL((init2d_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, time, U, t, g, ic));
+        return hipGetLastError();
+    }
     DG_DISPATCH_P(P, hipLaunchKernelGGL((init_state_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, time, U, t, g, ic));
     return hipGetLastError();
 }
 
 hipError_t launch_eval(int P, int mode, double time, const double* U, double* out, const KTables* t,
                        const KGeom& g, const KIC& ic, hipStream_t st) {
+    if (g.dim == 2) {
+        DG_DISPATCH_P(P, hipLaunchKernelGGL((eval2d_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, mode, time, U, out, t, g,
+                                            ic));
+        return hipGetLastError();
+    }
     DG_DISPATCH_P(P, hipLaunchKernelGGL((eval_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, mode, time, U, out, t, g, ic));
     return hipGetLastError();
 }
 
 hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables* t, const KGeom& g,
                             hipStream_t st) {
+    if (g.dim == 2) {
+        DG_DISPATCH_P(P, hipLaunchKernelGGL((wavespeed2d_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, U, out, t, g));
+        return hipGetLastError();
+    }
     DG_DISPATCH_P(P, hipLaunchKernelGGL((wavespeed_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, U, out, t, g));
     return hipGetLastError();
 }

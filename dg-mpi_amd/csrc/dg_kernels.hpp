@@ -23,11 +23,15 @@ struct KTables {
     double xq[2 * PMAX + 2];        // (2p+2)-point GL nodes
     double wq[2 * PMAX + 2];        // (2p+2)-point GL weights
     double xv[PMAX + 2];            // (p+2)-point GL nodes (volume and face points)
+    double BF[PMAX + 1][PMAX + 1];  // 2D faces: phi_i at the (p+1)-point segment rule
+    double wf[PMAX + 1];            // 2D faces: weights of that rule
     int minv_diag;                  // 1: minv is diagonal (Legendre)
 };
 
-// Structured periodic box and physics constants.
+// Structured periodic box and physics constants.  dim = 2: nx x ny box of
+// quadrilaterals (ne[2] = 1), 4 state variables.
 struct KGeom {
+    int dim;                        // 2 or 3
     int ne[3];                      // elements per direction
     unsigned E;                     // number of elements
     double lo[3], h[3];             // box corner, element widths
@@ -86,6 +90,7 @@ hipError_t launch_init_state(int P, double time, double* U, const KTables* t, co
 // mode 0: per-element squared L2 error vs exact solution at time t and the
 //         element volume (sum of w |J|) -> out[E][6]
 // mode 1: per-element integrals (rho, rho u, rho v, rho w, rho E, KE) -> out[E][6]
+//         (2D: rho, rho u, rho v, rho E, KE, 0)
 hipError_t launch_eval(int P, int mode, double time, const double* U, double* out,
                        const KTables* t, const KGeom& g, const KIC& ic, hipStream_t st);
 // mapped box: per-element mass matrices on the (2p+2)^3 rule, inverted in

@@ -48,7 +48,8 @@ static int read_raw(dg_solver* s, const std::string& data) {
     d.read((char*)&layout, 4);
     dg_info info;
     dg_get_info(s, &info);
-    if (!d || std::memcmp(magic, "DGAMDSOL", 8) || E != info.num_elems || nb != info.num_basis || ns != 5) {
+    if (!d || std::memcmp(magic, "DGAMDSOL", 8) || E != info.num_elems || nb != info.num_basis ||
+        ns != info.num_state_vars) {
         std::fprintf(stderr, "%s does not match the case\n", data.c_str());
         return 1;
     }
@@ -109,13 +110,14 @@ int main(int argc, char** argv) {
     const double time = info.time;
     std::printf("time=%g\n", time);
     std::printf("solver time = %f\n", time);
-    double err[5];
+    double err[5] = {0., 0., 0., 0., 0.};
     if (dg_l2_error(s, time, err) != DG_OK) {
         std::fprintf(stderr, "error: %s\n", dg_last_error(s));
         dg_destroy(s);
         return 1;
     }
-    for (int i = 0; i < 5; ++i) std::printf("norm_total_error(%d)=%.*e\n", i, DBL_DECIMAL_DIG, err[i]);
+    for (int i = 0; i < info.num_state_vars; ++i)
+        std::printf("norm_total_error(%d)=%.*e\n", i, DBL_DECIMAL_DIG, err[i]);
     dg_destroy(s);
     return 0;
 }

@@ -21,7 +21,8 @@ struct dg_solver {
     dg::KGeom geo;
     dg::KIC ic;
     int P = 0;
-    size_t ncoef = 0;                  // E * nb * 5
+    int ns = 5;                        // state variables: dim + 2
+    size_t ncoef = 0;                  // E * nb * ns
     double dt = 0.0;                   // case time step (0 until known)
     double time = 0.0;
     int device = 0;
@@ -235,8 +236,9 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         s->geo = dg::make_geometry(cfg);
         s->ic = dg::make_ic(cfg);
         s->dt = cfg.timestep;
-        const size_t nb = (size_t)(s->P + 1) * (s->P + 1) * (s->P + 1);
-        s->ncoef = (size_t)s->geo.E * nb * dg::NVAR;
+        const size_t nb = (size_t)(s->P + 1) * (s->P + 1) * (cfg.dim == 3 ? (s->P + 1) : 1);
+        s->ns = cfg.dim + 2;
+        s->ncoef = (size_t)s->geo.E * nb * s->ns;
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
             throw DeviceError("no HIP device available");
@@ -326,8 +328,9 @@ int dg_get_info(const dg_solver* s, dg_info* info) {
         const auto& c = s->cfg;
         info->num_elems = s->geo.E;
         info->order = c.order;
-        info->num_basis = (c.order + 1) * (c.order + 1) * (c.order + 1);
-        info->num_state_vars = dg::NVAR;
+        info->num_basis = (c.order + 1) * (c.order + 1) * (c.dim == 3 ? c.order + 1 : 1);
+        info->num_state_vars = s->ns;
+        info->dim = c.dim;
         for (int d = 0; d < 3; ++d) {
             info->nelem[d] = c.nelem[d];
             info->lower[d] = c.lower[d];
@@ -355,6 +358,25 @@ int dg_get_mesh(const dg_solver* s, long* num_nodes, double* node_coords, long* 
             *num_nodes = (long)(s->mesh_xyz.size() / 3);
             if (node_coords) std::memcpy(node_coords, s->mesh_xyz.data(), s->mesh_xyz.size() * sizeof(double));
             if (elem_nodes) std::memcpy(elem_nodes, s->mesh_e2n.data(), s->mesh_e2n.size() * sizeof(long));
+            return;
+        }
+        if (c.dim == 2) {
+            // quadrilaterals: (nx+1)(ny+1) grid nodes i + (nx+1) j, element
+            // e = ix + nx iy with node 2j + i of its frame
+            const long nx = c.nelem[0], ny = c.nelem[1], Nx = nx + 1;
+            *num_nodes = Nx * (ny + 1);
+            if (node_coords)
+                for (long j = 0; j <= ny; ++j)
+                    for (long i = 0; i <= nx; ++i) {
+                        double* x = node_coords + 2 * (i + Nx * j);
+                        x[0] = c.lower[0] + (c.upper[0] - c.lower[0]) * (double)i / (double)nx;
+                        x[1] = c.lower[1] + (c.upper[1] - c.lower[1]) * (double)j / (double)ny;
+                    }
+            if (elem_nodes)
+                for (long j = 0; j < ny; ++j)
+                    for (long i = 0; i < nx; ++i)
+                        for (int n = 0; n < 4; ++n)
+                            elem_nodes[4 * (i + nx * j) + n] = (i + (n & 1)) + Nx * (j + (n >> 1));
             return;
         }
         // the box: (n+1)^3 grid nodes i + (nx+1)(j + (ny+1) k), the warp
@@ -388,7 +410,8 @@ int dg_init_state(dg_solver* s) {
             // CFL rule (new): dt = cfl * min(h) / max(|u| + c) of the initial state
             const double amax = max_wavespeed(s, s->U0);
             if (!(amax > 0.0)) throw std::domain_error("zero wave speed: cannot derive dt from Stepper.cfl");
-            const double hmin = std::fmin(s->geo.h[0], std::fmin(s->geo.h[1], s->geo.h[2]));
+            const double hmin = s->cfg.dim == 2 ? std::fmin(s->geo.h[0], s->geo.h[1])
+                                                : std::fmin(s->geo.h[0], std::fmin(s->geo.h[1], s->geo.h[2]));
             s->dt = s->cfg.cfl * hmin / amax;
         }
     });
@@ -402,11 +425,11 @@ int dg_set_state(dg_solver* s, const double* host, int layout) {
         if (layout == DG_LAYOUT_RIGHT) {
             hip_check(hipMemcpy(s->U0, host, s->ncoef * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
         } else if (layout == DG_LAYOUT_LEFT) {
-            const size_t E = s->geo.E, nb = s->ncoef / (E * dg::NVAR);
+            const size_t E = s->geo.E, ns = s->ns, nb = s->ncoef / (E * ns);
             std::vector<double> t(s->ncoef);
-            for (size_t v = 0; v < dg::NVAR; ++v)
+            for (size_t v = 0; v < ns; ++v)
                 for (size_t b = 0; b < nb; ++b)
-                    for (size_t e = 0; e < E; ++e) t[(e * nb + b) * dg::NVAR + v] = host[(v * nb + b) * E + e];
+                    for (size_t e = 0; e < E; ++e) t[(e * nb + b) * ns + v] = host[(v * nb + b) * E + e];
             hip_check(hipMemcpy(s->U0, t.data(), s->ncoef * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
         } else {
             throw std::invalid_argument("unknown layout");
@@ -424,12 +447,12 @@ int dg_get_state(const dg_solver* cs, double* host, int layout) {
         if (layout == DG_LAYOUT_RIGHT) {
             hip_check(hipMemcpy(host, s->U0, s->ncoef * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
         } else if (layout == DG_LAYOUT_LEFT) {
-            const size_t E = s->geo.E, nb = s->ncoef / (E * dg::NVAR);
+            const size_t E = s->geo.E, ns = s->ns, nb = s->ncoef / (E * ns);
             std::vector<double> t(s->ncoef);
             hip_check(hipMemcpy(t.data(), s->U0, s->ncoef * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
-            for (size_t v = 0; v < dg::NVAR; ++v)
+            for (size_t v = 0; v < ns; ++v)
                 for (size_t b = 0; b < nb; ++b)
-                    for (size_t e = 0; e < E; ++e) host[(v * nb + b) * E + e] = t[(e * nb + b) * dg::NVAR + v];
+                    for (size_t e = 0; e < E; ++e) host[(v * nb + b) * E + e] = t[(e * nb + b) * ns + v];
         } else {
             throw std::invalid_argument("unknown layout");
         }
@@ -514,11 +537,16 @@ int dg_l2_error(dg_solver* s, double t, double err[5]) {
         } else {
             double wsum = 0.0;
             for (int a = 0; a < s->P + 2; ++a)
-                for (int b = 0; b < s->P + 2; ++b)
+                for (int b = 0; b < s->P + 2; ++b) {
+                    if (s->cfg.dim == 2) {
+                        wsum += s->tab.w[a] * s->tab.w[b];
+                        continue;
+                    }
                     for (int c = 0; c < s->P + 2; ++c) wsum += s->tab.w[a] * s->tab.w[b] * s->tab.w[c];
+                }
             vol = (double)s->geo.E * (wsum * s->geo.jdet);
         }
-        for (int v = 0; v < dg::NVAR; ++v) err[v] = std::pow(csum(h, s->geo.E, 6, v) / vol, 1. / 2);
+        for (int v = 0; v < s->ns; ++v) err[v] = std::pow(csum(h, s->geo.E, 6, v) / vol, 1. / 2);
     });
 }
 

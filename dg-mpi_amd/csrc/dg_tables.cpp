@@ -120,6 +120,14 @@ KTables make_tables(const CaseConfig& c) {
         }
         t.w[a] = w[a];
     }
+    // (p+1)-point segment rule: the face quadrature of quadrilaterals
+    // (solver/helpers.cpp:225, quadrature/segment.cpp:424-426: order 2p+1)
+    gl_rule(n, x, w);
+    for (int a = 0; a < n; ++a) {
+        basis_1d(c.basis, p, x[a], v, g);
+        for (int i = 0; i < n; ++i) t.BF[a][i] = v[i];
+        t.wf[a] = w[a];
+    }
     basis_1d(c.basis, p, -1.0, v, g);
     for (int i = 0; i < n; ++i) t.phim[i] = v[i];
     basis_1d(c.basis, p, 1.0, v, g);
@@ -177,12 +185,23 @@ KGeom make_geometry(const CaseConfig& c) {
         E *= c.nelem[d];
     }
     g.E = (unsigned)E;
-    g.jdet = (0.5 * g.h[0]) * (0.5 * g.h[1]) * (0.5 * g.h[2]);
-    g.jinv = 1.0 / g.jdet;
-    for (int d = 0; d < 3; ++d) {
-        g.gvol[d] = (2.0 / g.h[d]) * g.jdet;
-        const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-        g.farea[d] = (0.5 * g.h[t1]) * (0.5 * g.h[t2]);
+    g.dim = c.dim;
+    if (c.dim == 2) {
+        // quadrilaterals: |J| = hx hy / 4, |N| = half the edge length
+        g.jdet = (0.5 * g.h[0]) * (0.5 * g.h[1]);
+        g.jinv = 1.0 / g.jdet;
+        g.gvol[0] = (2.0 / g.h[0]) * g.jdet;
+        g.gvol[1] = (2.0 / g.h[1]) * g.jdet;
+        g.farea[0] = 0.5 * g.h[1];
+        g.farea[1] = 0.5 * g.h[0];
+    } else {
+        g.jdet = (0.5 * g.h[0]) * (0.5 * g.h[1]) * (0.5 * g.h[2]);
+        g.jinv = 1.0 / g.jdet;
+        for (int d = 0; d < 3; ++d) {
+            g.gvol[d] = (2.0 / g.h[d]) * g.jdet;
+            const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+            g.farea[d] = (0.5 * g.h[t1]) * (0.5 * g.h[t2]);
+        }
     }
     g.gamma = c.gamma;
     g.R = c.gas_constant;
@@ -209,6 +228,10 @@ KGeom make_geometry(const CaseConfig& c) {
     }
     // a given mesh is traversed in its own element order
     if (c.mesh_input) g.tile[0] = g.tile[1] = g.tile[2] = 0;
+    if (c.dim == 2) {
+        g.tile[0] = g.tile[1] = g.tile[2] = 0;
+        g.xcd_remap = 0;
+    }
     return g;
 }
 

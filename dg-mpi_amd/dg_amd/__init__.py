@@ -51,7 +51,7 @@ class Info(C.Structure):
                 ("gamma", C.c_double), ("gas_constant", C.c_double),
                 ("timestep", C.c_double), ("end_time", C.c_double), ("time", C.c_double),
                 ("num_iterations", C.c_int), ("stepper", C.c_int), ("flux", C.c_int),
-                ("basis", C.c_int), ("ic", C.c_int)]
+                ("basis", C.c_int), ("ic", C.c_int), ("dim", C.c_int)]
 
 
 _lib = None
@@ -182,8 +182,9 @@ class Solver:
         """(node coordinates [n][3], element nodes [E][8]) of the case."""
         n = C.c_long()
         self._check(lib().dg_get_mesh(self._h, C.byref(n), None, None))
-        xyz = np.zeros((n.value, 3))
-        e2n = np.zeros((self.info.num_elems, 8), dtype=np.int64)
+        dim = self.info.dim
+        xyz = np.zeros((n.value, dim))
+        e2n = np.zeros((self.info.num_elems, 2 ** dim), dtype=np.int64)
         self._check(lib().dg_get_mesh(self._h, C.byref(n), _ptr(xyz), e2n.ctypes.data_as(C.POINTER(C.c_long))))
         return xyz, e2n
 
@@ -264,14 +265,16 @@ def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(
               gas_constant=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0),
               warp=0.0, normals="reference", box=True):
     """TOML text of a periodic-box case (keys of the reference input decks;
-    warp > 0 gives the non-affine box).  box=False: the case of a mesh
+    warp > 0 gives the non-affine box; two entries in nelem, lower and upper
+    give the 2D case, Physics.dim = 2).  box=False: the case of a mesh
     passed to Solver(..., mesh=...), whose [Mesh] holds only the normals
     placement and the periodic cell (lower, upper) of the exact solutions."""
     st = f"timestep = {timestep!r}" if timestep is not None else f"cfl = {cfl!r}"
     arr = lambda v: "[" + ", ".join(repr(float(x)) for x in v) + "]"
+    dim = len(nelem)            # 2: a box of quadrilaterals (basis *Quad, 4 state variables)
     mesh = (f"""type = "PeriodicBox"
 warp = {float(warp)!r}
-nelem = [{nelem[0]}, {nelem[1]}, {nelem[2]}]
+nelem = [{", ".join(str(int(n)) for n in nelem)}]
 """ if box else "")
     return f"""[Mesh]
 {mesh}normals = "{normals}"
@@ -289,7 +292,7 @@ end = {float(end)!r}
 
 [Physics]
 name = "Euler"
-dim = 3
+dim = {dim}
 gamma = {gamma!r}
 GasConstant = {gas_constant!r}
 convective_flux_fcn = "{flux}"

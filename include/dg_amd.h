@@ -62,7 +62,10 @@ typedef struct {
     int basis;               /* 0 LagrangeHex, 1 LegendreHex                 */
     int ic;                  /* 0 Uniform, 1 Sphere, 2 IsentropicVortex,
                                 3 ExtrudedIsentropicVortex, 4 GaussianBump,
-                                5 TaylorGreen                                */
+                                5 TaylorGreen, 6 Gaussian (2D)               */
+    int dim;                 /* 3: hexahedra, 5 state variables; 2:
+                                quadrilaterals, 4 state variables (rho,
+                                rho u, rho v, rho E), nelem[2] = 1           */
 } dg_info;
 
 /* Parse a TOML case (the keys of src/exec/main.cpp:38-83,
@@ -108,7 +111,8 @@ int dg_get_info(const dg_solver* s, dg_info* info);
  * node i + (nx+1)(j + (ny+1)k), element ix + nx(iy + ny iz)) -- the arrays
  * the reference's Writer stores (src/io/writer.cpp:68-98).  Call with NULL
  * arrays to get num_nodes; then node_coords holds num_nodes*3 doubles and
- * elem_nodes num_elems*8 longs. */
+ * elem_nodes num_elems*8 longs.  In 2D: [num_nodes][2] and [num_elems][4]
+ * (node 2j + i, element ix + nx iy). */
 int dg_get_mesh(const dg_solver* s, long* num_nodes, double* node_coords, long* elem_nodes);
 
 /* L2 projection of the initial condition on (2p+2)^3 Gauss points, time
@@ -147,12 +151,13 @@ int dg_solve(dg_solver* s);
 
 /* Per-variable L2 error against the exact solution at time t, normalised
  * by the domain volume, on (2p+2)^3 points (get_error with ord = 2,
- * src/post/main.cpp:30-201). */
+ * src/post/main.cpp:30-201).  2D: (2p+2)^2 points, err[0..3]. */
 int dg_l2_error(dg_solver* s, double t, double err[5]);
 
 /* Domain integrals of rho, rho u, rho v, rho w, rho E and of the kinetic
  * energy 1/2 rho |u|^2 on (2p+2)^3 points, compensated FP64 sums (no
- * reference counterpart; conservation and kinetic-energy diagnostics). */
+ * reference counterpart; conservation and kinetic-energy diagnostics).
+ * 2D: rho, rho u, rho v, rho E, kinetic energy, 0. */
 int dg_integrals(dg_solver* s, double out[6]);
 
 /* Largest |u| + c over the volume quadrature points of the state (the

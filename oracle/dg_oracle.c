@@ -1291,6 +1291,17 @@ void orc_step(orc_solver* s, int stepper, double dt, int nsteps) {
     }
 }
 
+void orc_flux_interior_n(double gamma, int n, const double* U, double* Fdir) {
+    for (int i = 0; i < n; ++i) orc_flux_interior(gamma, U + (size_t)i * NS, Fdir + (size_t)i * 3 * NS);
+}
+
+void orc_flux_numerical_n(int flux, double gamma, int n, const double* UL, const double* UR,
+                          const double* N, double* F) {
+    for (int i = 0; i < n; ++i)
+        orc_flux_numerical(flux, gamma, UL + (size_t)i * NS, UR + (size_t)i * NS, N + (size_t)i * 3,
+                           F + (size_t)i * NS);
+}
+
 /* get_error: src/post/main.cpp:30-201 (ord = 2, normalised by volume) */
 void orc_l2_error(const orc_solver* s, double t, double* err) {
     int nb = s->nb;

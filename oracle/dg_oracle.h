@@ -76,6 +76,12 @@ double orc_maxwavespeed(double gamma, const double* U);
 void   orc_flux_interior(double gamma, const double* U, double* Fdir);
 void   orc_flux_numerical(int flux, double gamma, const double* UL,
                           const double* UR, const double* N, double* F);
+/* the same two functions over n points (U, UL, UR [n][5], N [n][3],
+ * F [n][5]; Fdir [n][3][5]) -- batch entry points for the 2D oracle, which
+ * embeds (rho, rho u, rho v, rho E) as (rho, rho u, rho v, 0, rho E) */
+void   orc_flux_interior_n(double gamma, int n, const double* U, double* Fdir);
+void   orc_flux_numerical_n(int flux, double gamma, int n, const double* UL,
+                            const double* UR, const double* N, double* F);
 void   orc_ic_eval(const orc_params* prm, const double* x, double t, double* U);
 
 /* ---- solver ---- */

@@ -60,6 +60,8 @@ def lib():
             "orc_maxwavespeed": (C.c_double, [C.c_double, _dp]),
             "orc_flux_interior": (None, [C.c_double, _dp, _dp]),
             "orc_flux_numerical": (None, [C.c_int, C.c_double, _dp, _dp, _dp, _dp]),
+            "orc_flux_interior_n": (None, [C.c_double, C.c_int, _dp, _dp]),
+            "orc_flux_numerical_n": (None, [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
             "orc_ic_eval": (None, [C.POINTER(Params), _dp, C.c_double, _dp]),
             "orc_create": (C.c_void_p, [C.POINTER(Params)]),
             "orc_create_mesh": (C.c_void_p, [C.POINTER(Params), C.c_long, _dp, C.c_long,

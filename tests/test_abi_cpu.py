@@ -40,10 +40,12 @@ BASE = dg_amd.case_toml(order=2, nelem=(2, 2, 2), timestep=0.01)
 
 @pytest.mark.parametrize("edit,status,needle", [
     (("convective_flux_fcn = \"Roe\"", "convective_flux_fcn = \"Rusanov\""), 1, "Rusanov is not a valid choice"),
-    (("basis = \"LegendreHex\"", "basis = \"LegendreQuad\""), 4, "LegendreQuad"),
+    (("basis = \"LegendreHex\"", "basis = \"LegendreQuad\""), 1, "does not match Physics.dim = 3"),
+    (("basis = \"LegendreHex\"", "basis = \"LagrangeSeg\""), 4, "LagrangeSeg"),
     (("basis = \"LegendreHex\"", "basis = \"Spline\""), 1, "not a valid choice"),
     (("type = \"RK4\"", "type = \"SSPRK3\""), 1, "SSPRK3 is not a valid choice"),
-    (("dim = 3", "dim = 2"), 4, "dim = 2"),
+    (("dim = 3", "dim = 2"), 1, "does not match Physics.dim = 2"),
+    (("dim = 3", "dim = 4"), 1, "Physics.dim must be 2 or 3"),
     (("order = 2", "order = 6"), 4, "no GPU kernel"),
     (("order = 2", "order = 9"), 1, "Legendre polynomial P = 9"),
     (("timestep = 0.01", "timestep = \"x\""), 1, "must be a number"),
@@ -154,3 +156,37 @@ def test_mesh_input_is_checked_before_the_device():
     assert rc == 1 and "periodic box only" in msg, msg
     rc, msg = _create_mesh(MESH_CASE.replace("timestep = 0.01", "cfl = 0.1"), xyz, e2n, ifc)
     assert rc == 1 and "timestep is required" in msg, msg
+
+
+BASE2 = dg_amd.case_toml(order=2, basis="LagrangeQuad", flux="HLLC", nelem=(3, 2), lower=(-5, -5), upper=(5, 5),
+                         timestep=0.01, ic="IsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0))
+
+
+@pytest.mark.parametrize("edit,status,needle", [
+    (("nelem = [3, 2]", "nelem = [3, 2, 2]"), 1, "Mesh.nelem must hold 2 integers"),
+    (("lower = [-5.0, -5.0]", "lower = [-5.0, -5.0, -5.0]"), 1, "Mesh.lower must hold 2 numbers"),
+    (("warp = 0.0", "warp = 0.1"), 4, "dim = 2"),
+    (("name = \"IsentropicVortex\"", "name = \"TaylorGreen\""), 4, "3D initial condition"),
+    (("data = [1.0, 1.0, 1.0, 1.0, 5.0]", "data = [1.0, 1.0]"), 1, "needs 5 values"),
+    (("GasConstant = 1.0", "GasConstant = 287.05"), 1, "GasConstant = 1"),
+    (("basis = \"LagrangeQuad\"", "basis = \"LagrangeHex\""), 1, "does not match Physics.dim = 2"),
+])
+def test_input_errors_2d(edit, status, needle):
+    """The dim = 2 case (Physics<2>, quadrilateral bases and ICs,
+    src/physics/base/base.cpp:77-88) is checked like the 3D one."""
+    text = BASE2.replace(*edit)
+    assert text != BASE2
+    rc, msg = _create(text)
+    assert rc == status, msg
+    assert needle in msg, msg
+
+
+def test_valid_2d_cases_reach_the_device():
+    from dg_amd import hip
+    want = 0 if hip.device_count() > 0 else 2
+    for ic, data in (("Uniform", (1.0, 0.1, 0.2, 2.5)), ("Gaussian", (1.0, 0.5, 0.5, 2.0)),
+                     ("IsentropicVortex", (1.0, 1.0, 1.0, 1.0, 5.0))):
+        text = BASE2.replace('name = "IsentropicVortex"', f'name = "{ic}"').replace(
+            "data = [1.0, 1.0, 1.0, 1.0, 5.0]", "data = [" + ", ".join(repr(float(x)) for x in data) + "]")
+        rc, msg = _create(text)
+        assert rc == want, msg
