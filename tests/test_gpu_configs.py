@@ -1,0 +1,98 @@
+"""The measured workloads at their own sizes, checked on the GPU.
+
+* 8^3 p=2 Legendre Roe RK4 at CFL 0.1 for 100 steps (z-extruded vortex on
+  [-5,5]^3): final state and L2 error against the oracle run to the same
+  step.
+* 64^3 p=1 Gaussian density bump (amplitude 0.2, width 1) carried by a
+  uniform flow of unit speed in a seeded random direction, 50 RK4 steps:
+  mass, momentum and energy conserved to round-off; the error against the
+  advected bump drops at the p+1 = 2 rate between 32^3 and 64^3.
+* 96^3 p=4 inviscid Taylor-Green vortex at Mach 0.1 on [0,2pi]^3 (110.6M
+  DOF per variable set), a few RK4 steps: conserved integrals to round-off,
+  zero mean momentum, the kinetic energy of the initial field and nearly
+  constant over the steps, finite state.
+Sizes beyond what the oracle finishes in seconds are checked through these
+size-independent properties.
+"""
+import numpy as np
+import pytest
+
+import dg_amd
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VORTEX = ("ExtrudedIsentropicVortex", (1.0, 1.0, 1.0, 1.0, 5.0))
+
+
+def test_vortex_8cube_p2_100_steps_against_the_oracle():
+    toml = dg_amd.case_toml(order=2, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), stepper="RK4", cfl=0.1,
+                            end=100.0, ic=VORTEX[0], ic_data=VORTEX[1])
+    with dg_amd.Solver(toml) as g:
+        g.init_state_from_fcn()
+        dt = g.timestep
+        o = O.Solver(O.make_params(order=2, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lo=(-5, -5, -5),
+                                   hi=(5, 5, 5), gamma=1.4, R=1.0, ic=VORTEX[0], ic_data=VORTEX[1]))
+        o.init_state()
+        assert abs(dt - 0.1 * (10 / 8) / o.max_wavespeed()) < 1e-14
+        print(f"dt = {dt!r}", flush=True)
+        g.take_time_step(dt, 100)
+        o.step("RK4", dt, 100)
+        U, R = g.get_state(), o.get_state()
+        assert np.max(np.abs(U - R)) / np.max(np.abs(R)) < 1e-10
+        t = g.time
+        assert abs(t - o.time) < 1e-12
+        eg, eo = g.get_error(t), o.l2_error(t)
+        assert np.allclose(eg, eo, rtol=1e-9, atol=1e-15)
+        print("L2 errors", eg, flush=True)
+        assert eg[0] < 1e-2
+
+
+def bump_toml(n):
+    v = np.random.default_rng(2).normal(size=3)
+    v /= np.linalg.norm(v)
+    data = (0.2, 1.0, v[0], v[1], v[2], 1.0, 0.0, 0.0, 0.0)
+    return dg_amd.case_toml(order=1, basis="LegendreHex", flux="Roe", nelem=(n, n, n), stepper="RK4",
+                            timestep=0.1 * 0.15625 / 2.5, end=1.0, ic="GaussianBump", ic_data=data)
+
+
+def test_gaussian_bump_64cube_p1_50_steps():
+    errs = {}
+    for n in (32, 64):
+        with dg_amd.Solver(bump_toml(n)) as g:
+            g.init_state_from_fcn()
+            I0 = g.integrals()
+            steps = 50 if n == 64 else 25
+            g.take_time_step(2 * 0.1 * 0.15625 / 2.5 if n == 32 else 0.0, steps)
+            I = g.integrals()
+            U = g.get_state()
+            assert np.all(np.isfinite(U))
+            assert np.all(np.abs(I[:5] - I0[:5]) < 1e-12 * np.abs(I0[:5]).max())
+            errs[n] = g.get_error()[0]
+            print(f"n={n} t={g.time} rho error {errs[n]:.4e}", flush=True)
+    # same final time on both meshes (dt doubled on 32^3); second order
+    rate = np.log2(errs[32] / errs[64])
+    assert 1.6 < rate < 3.0, rate
+
+
+def test_taylor_green_96cube_p4():
+    toml = dg_amd.case_toml(order=4, basis="LegendreHex", flux="Roe", nelem=(96, 96, 96),
+                            lower=(0, 0, 0), upper=(2 * np.pi, 2 * np.pi, 2 * np.pi), stepper="RK4", cfl=0.1,
+                            end=1.0, ic="TaylorGreen", ic_data=(0.1,))
+    with dg_amd.Solver(toml) as g:
+        assert g.shape == (96 ** 3, 125, 5)
+        g.init_state_from_fcn()
+        I0 = g.integrals()
+        ke = [I0[5]]
+        for _ in range(3):
+            g.take_time_step(0.0, 1)
+            ke.append(g.integrals()[5])
+        I = g.integrals()
+        print("integrals", I0, I, "dt", g.timestep, flush=True)
+        assert abs(I[0] - I0[0]) < 1e-12 * abs(I0[0])
+        assert abs(I[4] - I0[4]) < 1e-12 * abs(I0[4])
+        assert np.all(np.abs(I[1:4]) < 1e-9 * abs(I0[0]))
+        # well resolved and nearly incompressible: the kinetic energy moves
+        # only by pressure work and a little upwind dissipation
+        assert max(abs(k - ke[0]) for k in ke) < 1e-6 * ke[0], ke
+        assert ke[0] == pytest.approx(0.125 * (2 * np.pi) ** 3, rel=1e-6)
