@@ -40,8 +40,13 @@ def alg_flops_per_element(p):
     return 2 * (4 * V * S + 12 * V * T) + 60 * m ** 3 + 300 * m ** 2 + 4 * V * n ** 3
 
 
-def case_toml(ne, p=P, end=1.0, warp=0.0):
+def case_toml(ne, p=P, end=1.0, warp=0.0, dim=3):
     import dg_amd
+    if dim == 2:
+        # the 2D path: ne x ne quadrilaterals, the reference's 2D vortex
+        return dg_amd.case_toml(order=p, basis="LegendreQuad", flux="Roe", nelem=(ne, ne), lower=(-5., -5.),
+                                upper=(5., 5.), stepper="RK4", cfl=0.1, end=end, ic="IsentropicVortex",
+                                ic_data=(1.0, 1.0, 1.0, 1.0, 5.0))
     return dg_amd.case_toml(order=p, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne),
                             lower=(-5., -5., -5.), upper=(5., 5., 5.), stepper="RK4", cfl=0.1, end=end,
                             ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0), warp=warp)
@@ -94,6 +99,7 @@ def main():
     # other orders / the warped box: measurements beside the headline line
     ap.add_argument("--order", type=int, default=P, help=argparse.SUPPRESS)
     ap.add_argument("--warp", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--dim", type=int, default=3, choices=(2, 3), help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -106,14 +112,17 @@ def main():
     from dg_amd import hip
 
     assert hip.device_count() > 0, "bench.py needs a GPU"
-    ne, order = args.ne, args.order
-    s = dg_amd.Solver(case_toml(ne, order, warp=args.warp))
+    ne, order, dim = args.ne, args.order, args.dim
+    if dim == 2 and ne == NE:
+        ne = 1024                         # 2D: 1024^2 elements, the DOF count of the 3D headline at p=3
+    s = dg_amd.Solver(case_toml(ne, order, warp=args.warp, dim=dim))
     stream = hip.Stream()                 # default priority
     s.set_stream(stream.handle)
     s.init_state_from_fcn()
     dt = s.timestep
-    E = ne ** 3
-    nb = (order + 1) ** 3
+    E = ne ** dim
+    nb = (order + 1) ** dim
+    nvar = dim + 2
     dof = E * nb
 
     s.take_time_step(dt, args.warmup)
@@ -135,9 +144,9 @@ def main():
     err = s.get_error()
     finite = bool(np.all(np.isfinite(err)))
 
-    bytes_per_launch = ALG_BYTES_PER_DOF * dof
+    bytes_per_launch = ALG_BYTES_PER_DOF * nvar / 5 * dof
     achieved = bytes_per_launch / t_stage / 1e9
-    flops_launch = alg_flops_per_element(order) * E
+    flops_launch = alg_flops_per_element(order) * E if dim == 3 else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -150,27 +159,32 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (z-extruded isentropic vortex, generated on the GPU)",
-        "config": {"workload": f"euler3d_periodic_box_{ne}^3_p{order}_legendre_roe_rk4_cfl0.1"
-                               + (f"_warp{args.warp}" if args.warp else ""),
-                   "elements": E, "order": order, "dof": dof, "variables": 5, "flux": "Roe",
-                   "basis": "LegendreHex", "stepper": "RK4", "dt": dt, "domain": "[-5,5]^3"},
+        "data": "synthetic (z-extruded isentropic vortex, generated on the GPU)" if dim == 3 else
+                "synthetic (2D isentropic vortex, generated on the GPU)",
+        "config": {"workload": (f"euler3d_periodic_box_{ne}^3_p{order}_legendre_roe_rk4_cfl0.1"
+                                + (f"_warp{args.warp}" if args.warp else "")) if dim == 3 else
+                   f"euler2d_periodic_box_{ne}^2_p{order}_legendre_roe_rk4_cfl0.1",
+                   "elements": E, "order": order, "dof": dof, "variables": nvar, "flux": "Roe",
+                   "basis": "LegendreHex" if dim == 3 else "LegendreQuad", "stepper": "RK4", "dt": dt,
+                   "domain": "[-5,5]^3" if dim == 3 else "[-5,5]^2"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if (order == P and ne == NE and not args.warp) else None},
-        "kernel": {"name": f"stage_kernel<{order},Roe{',mapped' if args.warp else ''}>",
+                     "traffic": measured_traffic() if (order == P and ne == NE and not args.warp and dim == 3)
+                     else None},
+        "kernel": {"name": f"stage_kernel<{order},Roe{',mapped' if args.warp else ''}>" if dim == 3 else
+                   f"stage2d_kernel<{order},Roe>",
                    "avg_launch_ms": ms_per_step / 4,
                    "launches": 4 * args.steps, "alg_bytes_per_launch": bytes_per_launch,
                    "alg_flops_per_launch": flops_launch,
-                   "fp64_tflops": flops_launch / t_stage / 1e12,
-                   "fp64_frac": flops_launch / t_stage / 1e12 / FP64_PEAK_TFLOPS},
+                   "fp64_tflops": flops_launch / t_stage / 1e12 if flops_launch else None,
+                   "fp64_frac": flops_launch / t_stage / 1e12 / FP64_PEAK_TFLOPS if flops_launch else None},
         "l2_error_vs_exact": [float(x) for x in err],
         "finite": finite,
         "wall_s": wall,
     }
     s.close()
     stream.destroy()
-    if not args.no_cpu:
+    if not args.no_cpu and dim == 3:
         out["cpu_baseline"] = cpu_baseline()
     else:
         out["cpu_baseline"] = None
