@@ -128,6 +128,20 @@ def vortex2d(ref):
     for name, vals in re.findall(r"(\w+)\s*=\s*np\.array\(\[(.*?)\]\)", src, re.S):
         nums = re.findall(r"[-+]?[0-9]*\.?[0-9]+(?:[eE][-+]?[0-9]+)?", vals)
         out["series"][name] = [float(x) for x in nums]
+    # the active series of the study (the last assignment of each name, in
+    # uncommented lines), as written, without the entries marked "HOLD FOR
+    # NOW" (repeats of the entry before them)
+    out["settings_source"] = ("examples/isentropic_vortex/input.toml:1-23 (LagrangeQuad, HLLC, RK4, dt 0.001, "
+                              "end 1.0, IsentropicVortex [1,1,1,1,5], GasConstant 1)")
+    out["mesh"] = ("periodic N x N quadrilaterals on [-5,5]^2 (the reference's box_NxN.h5 mesh package is not "
+                   "shipped; the domain is inferred from these values, which it reproduces to ~1e-7)")
+    live = "\n".join(l for l in src.splitlines() if not l.lstrip().startswith("#"))
+    for name, vals in re.findall(r"(\w+)\s*=\s*\(?np\.array\(\[(.*?)\]\)", live, re.S):
+        if name == "Nelem":
+            out["nelem_1d"] = [int(float(x)) for x in re.findall(r"[0-9]+\.?[0-9]*", vals)]
+        elif re.fullmatch(r"err_p\d", name):
+            keep = [l for l in vals.splitlines() if "HOLD" not in l]
+            out[name] = re.findall(r"[-+]?[0-9]*\.?[0-9]+(?:[eE][-+]?[0-9]+)?", "\n".join(keep))
     return out
 
 
