@@ -90,6 +90,25 @@ def cpu_baseline(budget_s=20.0):
                       f"(p=3 LegendreHex, Roe, RK4), {steps} RK4 steps = {4 * steps} stages in {el:.1f} s"}
 
 
+def io_bench(s, path):
+    """Time dg_write_solution / dg_read_solution (data.h5, src/io/writer.cpp,
+    src/solver/base.cpp:233-279) on the workload's state, device <-> file."""
+    from dg_amd import h5
+    import numpy as np
+    nbytes = int(np.prod(s.shape)) * 8
+    rec = {}
+    for name, fn in (("write", lambda: s.write(path)), ("read", lambda: s.read_in_coefficients(path))):
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        rec[name] = {"s": best, "GB/s": nbytes / best / 1e9}
+    print(json.dumps({"io": rec, "state_bytes": nbytes, "file_bytes": os.path.getsize(path),
+                      "header": h5.read_header(path)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +119,8 @@ def main():
     ap.add_argument("--order", type=int, default=P, help=argparse.SUPPRESS)
     ap.add_argument("--warp", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--dim", type=int, default=3, choices=(2, 3), help=argparse.SUPPRESS)
+    # data.h5 write / read-back of the workload's state (file path)
+    ap.add_argument("--io", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -125,6 +146,9 @@ def main():
     nvar = dim + 2
     dof = E * nb
 
+    if args.io:
+        io_bench(s, args.io)
+        return
     s.take_time_step(dt, args.warmup)
     hip.synchronize()
     ev0, ev1 = hip.Event(), hip.Event()
