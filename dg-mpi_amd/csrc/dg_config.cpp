@@ -312,16 +312,13 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
                          "(src/physics/euler/functions.cpp:107)");
 
     // [Mesh]
-    if (mesh_input && dim == 2)
-        throw UnsupportedError("quadrilateral meshes from arrays or files are not on this path "
-                               "(Physics.dim = 2 takes Mesh.type = \"PeriodicBox\")");
     if (mesh_input) {
         // the mesh comes with the call (dg_create_mesh); lower / upper, if
         // given, are the periodic cell of the advected exact solutions
         std::vector<double> lo = get_num_array(d, "Mesh", "lower", false);
         std::vector<double> hi = get_num_array(d, "Mesh", "upper", false);
-        if (lo.size() == 3 && hi.size() == 3)
-            for (int i = 0; i < 3; ++i) {
+        if (lo.size() == (size_t)dim && hi.size() == (size_t)dim)
+            for (int i = 0; i < dim; ++i) {
                 c.lower[i] = lo[i];
                 c.upper[i] = hi[i];
             }
@@ -331,7 +328,7 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
         if (get_num(d, "Mesh", "warp", false, 0.0) != 0.0)
             throw InputError("Mesh.warp applies to the periodic box only");
         if (c.timestep <= 0.0) throw InputError("Stepper.timestep is required with a mesh (Stepper.cfl needs the box)");
-        if (c.order > 4)
+        if (dim == 3 && c.order > 4)
             throw UnsupportedError("meshes with order > 4: the per-element mass matrix inverse of the "
                                    "non-affine path is formed in LDS and holds at most (4+1)^3 basis functions");
         return c;

@@ -401,29 +401,31 @@ int dg_h5_read_mesh(const char* path, double* node_coords, long coords_capacity,
     });
 }
 
-int dg_h5_write_mesh(const char* path, long num_nodes, const double* node_coords, long num_elems,
+int dg_h5_write_mesh(const char* path, int dim, long num_nodes, const double* node_coords, long num_elems,
                      const long* elem_nodes, long num_ifaces, const int* iface_data) {
     return guarded([&] {
         if (!path || !node_coords || !elem_nodes || (num_ifaces > 0 && !iface_data))
             throw ArgError("dg_h5_write_mesh: NULL argument");
+        if (dim != 2 && dim != 3) throw ArgError("dg_h5_write_mesh: dim must be 2 or 3");
         if (num_nodes < 1 || num_elems < 1 || num_ifaces < 0) throw ArgError("dg_h5_write_mesh: bad sizes");
-        for (long i = 0; i < num_elems * 8; ++i)
+        const int nv = dim == 2 ? 4 : 8;
+        for (long i = 0; i < num_elems * nv; ++i)
             if (elem_nodes[i] < 0 || elem_nodes[i] >= num_nodes)
                 throw ArgError("dg_h5_write_mesh: element node index out of range");
         quiet_hdf5();
         std::vector<long> ifc((size_t)num_ifaces * 6);
         for (size_t i = 0; i < ifc.size(); ++i) ifc[i] = iface_data[i];
         Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose, "create " + errstr_path(path));
-        write_scalar(file, "Dimension", 3);
+        write_scalar(file, "Dimension", dim);
         write_scalar(file, "QOrder", 1);
         write_scalar(file, "nElem", num_elems);
         write_scalar(file, "nNode", num_nodes);
         write_scalar(file, "nIFace", num_ifaces);
-        write_scalar(file, "nNodePerElem", 8);
-        write_scalar(file, "nNodePerFace", 4);
+        write_scalar(file, "nNodePerElem", nv);
+        write_scalar(file, "nNodePerFace", nv / 2);
         write_scalar(file, "nBFaceGroup", 0);
-        write_2d(file, "NodeCoords", H5T_IEEE_F64LE, H5T_NATIVE_DOUBLE, num_nodes, 3, node_coords);
-        write_2d(file, "Elem2Nodes", H5T_STD_I64LE, H5T_NATIVE_LONG, num_elems, 8, elem_nodes);
+        write_2d(file, "NodeCoords", H5T_IEEE_F64LE, H5T_NATIVE_DOUBLE, num_nodes, dim, node_coords);
+        write_2d(file, "Elem2Nodes", H5T_STD_I64LE, H5T_NATIVE_LONG, num_elems, nv, elem_nodes);
         write_2d(file, "IFaceData", H5T_STD_I64LE, H5T_NATIVE_LONG, num_ifaces, 6, ifc.data());
     });
 }
@@ -442,20 +444,23 @@ int dg_create_from_mesh_file(const char* toml_text, const char* mesh_file, dg_so
         quiet_hdf5();
         Hid f(H5Fopen(file.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose, "open mesh file " + file);
         m = mesh_info(f);
-        if (m.dim != 3)
-            throw dg::UnsupportedError(file + ": Dimension = " + std::to_string(m.dim) +
-                                       " (quadrilateral meshes are not on this path; hexahedra only)");
-        if (m.qorder != 1 || m.num_nodes_per_elem != 8)
-            throw dg::UnsupportedError(file + ": QOrder = " + std::to_string(m.qorder) +
-                                       " with " + std::to_string(m.num_nodes_per_elem) +
-                                       " nodes per element; this path takes trilinear (QOrder 1, 8-node) hexahedra");
+        // the file's dimension is the case's (Mesh<dim>, mesh.cpp:28-60)
+        const int dim = dg::parse_case(toml_text, true).dim, nv = dim == 2 ? 4 : 8;
+        if (m.dim != dim)
+            throw dg::InputError(file + ": Dimension = " + std::to_string(m.dim) +
+                                 " does not match Physics.dim = " + std::to_string(dim));
+        if (m.qorder != 1 || m.num_nodes_per_elem != nv)
+            throw dg::UnsupportedError(file + ": QOrder = " + std::to_string(m.qorder) + " with " +
+                                       std::to_string(m.num_nodes_per_elem) + " nodes per element; this path takes " +
+                                       (dim == 2 ? "bilinear (QOrder 1, 4-node) quadrilaterals"
+                                                 : "trilinear (QOrder 1, 8-node) hexahedra"));
         if (m.num_bface_groups > 0)
             throw dg::UnsupportedError(file + ": boundary face groups are not on this path (periodic meshes only)");
-        xyz.resize((size_t)m.num_nodes * 3);
-        e2n.resize((size_t)m.num_elems * 8);
+        xyz.resize((size_t)m.num_nodes * dim);
+        e2n.resize((size_t)m.num_elems * nv);
         ifc.resize((size_t)m.num_ifaces * 6);
-        read_array(f, "NodeCoords", H5T_NATIVE_DOUBLE, m.num_nodes * 3, xyz.data());
-        read_array(f, "Elem2Nodes", H5T_NATIVE_LONG, m.num_elems * 8, e2n.data());
+        read_array(f, "NodeCoords", H5T_NATIVE_DOUBLE, m.num_nodes * dim, xyz.data());
+        read_array(f, "Elem2Nodes", H5T_NATIVE_LONG, m.num_elems * nv, e2n.data());
         read_array(f, "IFaceData", H5T_NATIVE_INT, m.num_ifaces * 6, ifc.data());
     });
     if (rc != DG_OK) return rc;

@@ -1889,6 +1889,10 @@ hipError_t launch_wavespeed(int P, const double* U, double* out, const KTables* 
 }
 
 hipError_t launch_mass(int P, double* imm, double* jmin, const KTables* t, const KGeom& g, hipStream_t st) {
+    if (g.dim == 2) {
+        DG_DISPATCH_P(P, hipLaunchKernelGGL((mass2d_kernel<PP>), dim3(g.E), dim3(WAVE), 0, st, imm, jmin, t, g));
+        return hipGetLastError();
+    }
     const dim3 grid(g.E), block(256);
     switch (P) {
     case 1: hipLaunchKernelGGL((mass_kernel<1>), grid, block, 0, st, imm, jmin, t, g); break;

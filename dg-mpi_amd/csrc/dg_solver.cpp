@@ -38,8 +38,8 @@ struct dg_solver {
     unsigned* frec = nullptr;          // general path: face records [E][6][2]
     double* nev = nullptr;             // general path: normal evaluation points
     dg::KTables* dtab = nullptr;       // device copy of tab
-    std::vector<double> mesh_xyz;      // given mesh: node coordinates [n][3]
-    std::vector<long> mesh_e2n;        // given mesh: element nodes [E][8]
+    std::vector<double> mesh_xyz;      // given mesh: node coordinates [n][dim]
+    std::vector<long> mesh_e2n;        // given mesh: element nodes [E][8] (2D: [E][4])
     std::string err;
 };
 
@@ -195,16 +195,18 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         need(toml_text, "toml_text");
         need(out, "out");
         dg::CaseConfig cfg = dg::parse_case(toml_text, mesh != nullptr);
+        // a given mesh: hexahedra of 8 nodes in 3D, quadrilaterals of 4 in 2D
+        const int dim = cfg.dim, nv = dim == 2 ? 4 : 8;
         if (mesh) {
             if (mesh->E < 1 || mesh->nnodes < 1 || mesh->F < 1 || mesh->E > 0x7fffffffL)
                 throw dg::InputError("mesh: empty or too large");
             need(mesh->coords, "coords");
             need(mesh->e2n, "elem_nodes");
             need(mesh->iface6, "iface_data");
-            for (long i = 0; i < mesh->E * 8; ++i)
+            for (long i = 0; i < mesh->E * nv; ++i)
                 if (mesh->e2n[i] < 0 || mesh->e2n[i] >= mesh->nnodes)
                     throw dg::InputError("mesh: element node index out of range");
-            for (long i = 0; i < mesh->nnodes * 3; ++i)
+            for (long i = 0; i < mesh->nnodes * dim; ++i)
                 if (!std::isfinite(mesh->coords[i])) throw dg::InputError("mesh: non-finite node coordinate");
             cfg.nelem[0] = (int)mesh->E;
             cfg.nelem[1] = cfg.nelem[2] = 1;
@@ -215,12 +217,13 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         std::vector<double> X;
         dg::FaceTables ft;
         if (mesh) {
-            X.resize((size_t)mesh->E * 24);
+            X.resize((size_t)mesh->E * nv * dim);
             for (long e = 0; e < mesh->E; ++e)
-                for (int n = 0; n < 8; ++n)
-                    for (int d = 0; d < 3; ++d)
-                        X[(size_t)e * 24 + n * 3 + d] = mesh->coords[mesh->e2n[e * 8 + n] * 3 + d];
-            ft = dg::mesh_face_tables(cfg, mesh->E, mesh->F, mesh->iface6);
+                for (int n = 0; n < nv; ++n)
+                    for (int d = 0; d < dim; ++d)
+                        X[((size_t)e * nv + n) * dim + d] = mesh->coords[mesh->e2n[e * nv + n] * dim + d];
+            ft = dim == 2 ? dg::mesh_face_tables_2d(cfg, mesh->E, mesh->F, mesh->iface6)
+                          : dg::mesh_face_tables(cfg, mesh->E, mesh->F, mesh->iface6);
         } else if (cfg.warp > 0.0) {
             X = dg::element_nodes(cfg);
             ft = dg::box_face_tables(cfg);
@@ -228,8 +231,8 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         s = new dg_solver();
         s->cfg = cfg;
         if (mesh) {
-            s->mesh_xyz.assign(mesh->coords, mesh->coords + mesh->nnodes * 3);
-            s->mesh_e2n.assign(mesh->e2n, mesh->e2n + mesh->E * 8);
+            s->mesh_xyz.assign(mesh->coords, mesh->coords + mesh->nnodes * dim);
+            s->mesh_e2n.assign(mesh->e2n, mesh->e2n + mesh->E * nv);
         }
         s->P = cfg.order;
         s->tab = dg::make_tables(cfg);
@@ -263,9 +266,11 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
             hip_check(hipMalloc(&s->frec, ft.frec.size() * sizeof(unsigned)), "hipMalloc(frec)");
             hip_check(hipMemcpy(s->frec, ft.frec.data(), ft.frec.size() * sizeof(unsigned), hipMemcpyHostToDevice),
                       "hipMemcpy(frec)");
-            hip_check(hipMalloc(&s->nev, ft.nev.size() * sizeof(double)), "hipMalloc(nev)");
-            hip_check(hipMemcpy(s->nev, ft.nev.data(), ft.nev.size() * sizeof(double), hipMemcpyHostToDevice),
-                      "hipMemcpy(nev)");
+            if (!ft.nev.empty()) {   // 3D only: 2D faces are straight
+                hip_check(hipMalloc(&s->nev, ft.nev.size() * sizeof(double)), "hipMalloc(nev)");
+                hip_check(hipMemcpy(s->nev, ft.nev.data(), ft.nev.size() * sizeof(double), hipMemcpyHostToDevice),
+                          "hipMemcpy(nev)");
+            }
             hip_check(hipMalloc(&s->imm, (size_t)s->geo.E * nb * nb * sizeof(double)), "hipMalloc(imm)");
             s->geo.X = s->X;
             s->geo.imm = s->imm;
@@ -355,7 +360,7 @@ int dg_get_mesh(const dg_solver* s, long* num_nodes, double* node_coords, long* 
         need(num_nodes, "num_nodes");
         const auto& c = s->cfg;
         if (!s->mesh_e2n.empty()) {
-            *num_nodes = (long)(s->mesh_xyz.size() / 3);
+            *num_nodes = (long)(s->mesh_xyz.size() / s->cfg.dim);
             if (node_coords) std::memcpy(node_coords, s->mesh_xyz.data(), s->mesh_xyz.size() * sizeof(double));
             if (elem_nodes) std::memcpy(elem_nodes, s->mesh_e2n.data(), s->mesh_e2n.size() * sizeof(long));
             return;

@@ -512,6 +512,46 @@ FaceTables mesh_face_tables(const CaseConfig& c, long E, long F, const int* ifac
     return ft;
 }
 
+FaceTables mesh_face_tables_2d(const CaseConfig& c, long E, long F, const int* iface6) {
+    (void)c;
+    // quadrilateral faces (shape.cpp:69-122): 0 (n0 n1, y-), 1 (n1 n3, x+),
+    // 2 (n3 n2, y+), 3 (n2 n0, x-); slot d*2 + side of the face's normal
+    // direction; sigma: +1 when the face parameter runs along the element's
+    // tangential coordinate, -1 against it; orientation 1 reverses it
+    auto dir = [](int f) { return (f == 1 || f == 3) ? 0 : 1; };
+    auto side = [](int f) { return (f == 1 || f == 2) ? 1 : 0; };
+    auto sigma = [](int f, int o) { return ((f <= 1) ? 1 : -1) * (o ? -1 : 1); };
+    FaceTables ft;
+    ft.frec.assign((size_t)E * 8, 0);
+    std::vector<char> seen((size_t)E * 4, 0);
+    for (long f = 0; f < F; ++f) {
+        const int* r = iface6 + f * 6;
+        const long eL = r[0], eR = r[3];
+        const int fL = r[1], oL = r[2], fR = r[4], oR = r[5];
+        if (eL < 0 || eL >= E || eR < 0 || eR >= E || fL < 0 || fL > 3 || fR < 0 || fR > 3)
+            throw InputError("IFaceData row " + std::to_string(f) + " is out of range");
+        if (oL < 0 || oL > 1 || oR < 0 || oR > 1)
+            throw InputError("IFaceData row " + std::to_string(f) + ": face orientation is not 0..1");
+        const int sL = dir(fL) * 2 + side(fL), sR = dir(fR) * 2 + side(fR);
+        if (seen[(size_t)eL * 4 + sL]++ || seen[(size_t)eR * 4 + sR]++)
+            throw InputError("IFaceData: an element face appears twice");
+        // the flux point with face parameter s_q sits at the left face's
+        // tangential grid point a and the right face's point b; b = a or
+        // m - 1 - a (the pairing of construct_face_states, base.cpp:287-327)
+        const unsigned rev = sigma(fL, oL) != sigma(fR, oR) ? 1u : 0u;
+        ft.frec[((size_t)eL * 4 + sL) * 2] = (unsigned)eR;
+        ft.frec[((size_t)eL * 4 + sL) * 2 + 1] = (unsigned)fR | (rev << 2) | (1u << 3);
+        ft.frec[((size_t)eR * 4 + sR) * 2] = (unsigned)eL;
+        ft.frec[((size_t)eR * 4 + sR) * 2 + 1] = (unsigned)fL | (rev << 2);
+    }
+    for (size_t i = 0; i < seen.size(); ++i)
+        if (!seen[i])
+            throw UnsupportedError("element " + std::to_string(i / 4) +
+                                   " has a face without a neighbour: boundary faces are not on this path "
+                                   "(periodic meshes only)");
+    return ft;
+}
+
 KIC make_ic(const CaseConfig& c) {
     KIC k;
     std::memset(&k, 0, sizeof(k));

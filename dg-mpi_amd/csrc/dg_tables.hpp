@@ -52,4 +52,11 @@ FaceTables box_face_tables(const CaseConfig& c);
 // element face must be an interior face (periodic, no boundary faces).
 FaceTables mesh_face_tables(const CaseConfig& c, long E, long F, const int* iface6);
 
+// The same for quadrilateral meshes: frec [E][4][2], slot d*2 + side
+// (d = 0: faces 3, 1; d = 1: faces 0, 2): neighbour element, then the
+// neighbour's face id (bits 0-1) | the face points run in opposite senses of
+// the two tangential coordinates (bit 2) | this element is the left one
+// (bit 3).  No normal table: the faces are straight.
+FaceTables mesh_face_tables_2d(const CaseConfig& c, long E, long F, const int* iface6);
+
 }  // namespace dg

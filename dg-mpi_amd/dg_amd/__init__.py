@@ -107,7 +107,8 @@ class Solver:
     def __init__(self, toml_text, mesh=None):
         """mesh: None for the periodic box of the TOML case, else (node
         coordinates [n][3], element nodes [E][8], interior faces [F][6])
-        in the layout of the reference's mesh file."""
+        in the layout of the reference's mesh file; for Physics.dim = 2,
+        coordinates [n][2] and element nodes [E][4]."""
         h = C.c_void_p()
         if mesh is None:
             rc = lib().dg_create(toml_text.encode(), C.byref(h))
@@ -115,9 +116,10 @@ class Solver:
             xyz = np.ascontiguousarray(mesh[0], dtype=np.float64)
             e2n = np.ascontiguousarray(mesh[1], dtype=np.int64)
             ifc = np.ascontiguousarray(mesh[2], dtype=np.int32)
-            if xyz.ndim != 2 or xyz.shape[1] != 3 or e2n.ndim != 2 or e2n.shape[1] != 8 or \
-                    ifc.ndim != 2 or ifc.shape[1] != 6:
-                raise ValueError("mesh arrays must be [n][3], [E][8] and [F][6]")
+            dim = xyz.shape[1] if xyz.ndim == 2 else 0
+            if dim not in (2, 3) or e2n.ndim != 2 or e2n.shape[1] != 2 ** dim or ifc.ndim != 2 or \
+                    ifc.shape[1] != 6:
+                raise ValueError("mesh arrays must be [n][3], [E][8] and [F][6] (2D: [n][2], [E][4], [F][6])")
             rc = lib().dg_create_mesh(toml_text.encode(), xyz.shape[0], _ptr(xyz), e2n.shape[0],
                                       e2n.ctypes.data_as(C.POINTER(C.c_long)), ifc.shape[0],
                                       ifc.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h))
@@ -179,7 +181,7 @@ class Solver:
         return self._info().num_iterations
 
     def mesh(self):
-        """(node coordinates [n][3], element nodes [E][8]) of the case."""
+        """(node coordinates [n][dim], element nodes [E][2^dim]) of the case."""
         n = C.c_long()
         self._check(lib().dg_get_mesh(self._h, C.byref(n), None, None))
         dim = self.info.dim
@@ -271,7 +273,8 @@ def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(
     placement and the periodic cell (lower, upper) of the exact solutions."""
     st = f"timestep = {timestep!r}" if timestep is not None else f"cfl = {cfl!r}"
     arr = lambda v: "[" + ", ".join(repr(float(x)) for x in v) + "]"
-    dim = len(nelem)            # 2: a box of quadrilaterals (basis *Quad, 4 state variables)
+    # 2: quadrilaterals (basis *Quad, 4 state variables)
+    dim = len(nelem) if box else len(lower)
     mesh = (f"""type = "PeriodicBox"
 warp = {float(warp)!r}
 nelem = [{", ".join(str(int(n)) for n in nelem)}]

@@ -60,7 +60,7 @@ def lib():
             "dg_h5_read_coefficients": [C.c_char_p, dp, C.c_long, C.c_int],
             "dg_h5_read_mesh_info": [C.c_char_p, C.POINTER(MeshInfo)],
             "dg_h5_read_mesh": [C.c_char_p, dp, C.c_long, lp, C.c_long, ip, C.c_long],
-            "dg_h5_write_mesh": [C.c_char_p, C.c_long, dp, C.c_long, lp, C.c_long, ip],
+            "dg_h5_write_mesh": [C.c_char_p, C.c_int, C.c_long, dp, C.c_long, lp, C.c_long, ip],
             "dg_create_from_mesh_file": [C.c_char_p, C.c_char_p, C.POINTER(vp)],
             "dg_write_solution": [vp, C.c_char_p],
             "dg_read_solution": [vp, C.c_char_p, C.c_int],
@@ -145,14 +145,16 @@ def read_mesh(path):
 
 
 def write_mesh(path, coords, elem_nodes, iface_data):
-    """A 3D trilinear hex mesh file in the layout of the reference's reader."""
+    """A mesh file in the layout of the reference's reader: trilinear
+    hexahedra (coords [n][3], elem_nodes [E][8]) or bilinear quadrilaterals
+    (coords [n][2], elem_nodes [E][4]); iface_data [F][6]."""
     xyz = np.ascontiguousarray(coords, dtype=np.float64)
     e2n = np.ascontiguousarray(elem_nodes, dtype=np.int64)
     ifc = np.ascontiguousarray(iface_data, dtype=np.int32)
-    if xyz.ndim != 2 or xyz.shape[1] != 3 or e2n.ndim != 2 or e2n.shape[1] != 8 or ifc.ndim != 2 or \
-            ifc.shape[1] != 6:
-        raise ValueError("mesh arrays must be [n][3], [E][8] and [F][6]")
-    _check(lib().dg_h5_write_mesh(os.fsencode(path), len(xyz), _dptr(xyz), len(e2n),
+    dim = xyz.shape[1] if xyz.ndim == 2 else 0
+    if dim not in (2, 3) or e2n.ndim != 2 or e2n.shape[1] != 2 ** dim or ifc.ndim != 2 or ifc.shape[1] != 6:
+        raise ValueError("mesh arrays must be [n][3], [E][8] and [F][6] (2D: [n][2], [E][4], [F][6])")
+    _check(lib().dg_h5_write_mesh(os.fsencode(path), dim, len(xyz), _dptr(xyz), len(e2n),
                                   e2n.ctypes.data_as(C.POINTER(C.c_long)), len(ifc),
                                   ifc.ctypes.data_as(C.POINTER(C.c_int))))
 

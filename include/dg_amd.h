@@ -91,7 +91,10 @@ int dg_create(const char* toml_text, dg_solver** out);
  * no [Mesh] keys other than normals = "reference" | "exact"; the element
  * metric is computed at every quadrature point (non-affine elements are
  * allowed) and the per-element inverse mass matrices are formed on the
- * device (orders 1-4). */
+ * device (orders 1-4).  With Physics.dim = 2 the mesh holds bilinear
+ * quadrilaterals: node_coords [num_nodes][2], elem_nodes [num_elems][4]
+ * (node 2j + i), face ids 0..3 and orientations 0..1 of
+ * src/numerics/basis/shape.cpp:69-122 (orders 1-5). */
 int dg_create_mesh(const char* toml_text, long num_nodes, const double* node_coords, long num_elems,
                    const long* elem_nodes, long num_ifaces, const int* iface_data, dg_solver** out);
 

@@ -76,10 +76,11 @@ int dg_h5_read_mesh_info(const char* path, dg_h5_mesh_info* info);
 int dg_h5_read_mesh(const char* path, double* node_coords, long coords_capacity, long* elem_nodes,
                     long elem_nodes_capacity, int* iface_data, long iface_capacity);
 
-/* Write a 3D mesh file of trilinear hexahedra (QOrder 1, 8 nodes per
- * element, 4 per face, no boundary face groups): the datasets the
+/* Write a mesh file of QOrder 1 elements without boundary face groups:
+ * dim 3, trilinear hexahedra (8 nodes per element, 4 per face); dim 2,
+ * bilinear quadrilaterals (4 per element, 2 per face).  The datasets the
  * reference's mesh reader opens, with the arrays of dg_create_mesh. */
-int dg_h5_write_mesh(const char* path, long num_nodes, const double* node_coords, long num_elems,
+int dg_h5_write_mesh(const char* path, int dim, long num_nodes, const double* node_coords, long num_elems,
                      const long* elem_nodes, long num_ifaces, const int* iface_data);
 
 /* dg_create_mesh with the arrays of a mesh file (the Mesh constructor,

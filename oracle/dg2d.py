@@ -279,3 +279,112 @@ class Box2D:
     def integrals(self):
         uh = np.einsum("qb,ebs->eqs", self.mval, self.U)
         return np.einsum("eqs,q->s", uh, self.mwts * self.jdet)
+
+
+# ---------------------------------------------------------------------------
+# Quadrilateral meshes given as the reference's mesh arrays (Mesh::read_mesh,
+# src/mesh/mesh.cpp:79-175): NodeCoords [n][2], Elem2Nodes [E][4] (node
+# 2j + i of the element's frame), IFaceData [F][6] = (elemL, faceL, orientL,
+# elemR, faceR, orientR).  Bilinear elements: per-point metric
+# (helpers.cpp:4-51, basis/tools.cpp:42-66), dense per-element mass
+# matrices (helpers.cpp:53-100), face points from the orientations
+# (base.cpp:287-327), normals of the left face N = (dy/ds, -dx/ds)
+# (shape.cpp:157-200; straight edges, so constant along the face).
+# ---------------------------------------------------------------------------
+
+def bilinear(X, pts):
+    """Physical points, Jacobian columns a_xi, a_eta [n][2] and |J| at
+    reference points pts [n][2] of an element with nodes X [4][2]."""
+    xi, eta = pts[:, 0], pts[:, 1]
+    x = np.zeros((len(pts), 2))
+    ax = np.zeros((len(pts), 2))
+    ae = np.zeros((len(pts), 2))
+    for n in range(4):
+        sx, se = 2 * (n & 1) - 1, 2 * (n >> 1) - 1
+        w = (1 + sx * xi) * (1 + se * eta) / 4.
+        dx = sx * (1 + se * eta) / 4.
+        de = se * (1 + sx * xi) / 4.
+        x += w[:, None] * X[n]
+        ax += dx[:, None] * X[n]
+        ae += de[:, None] * X[n]
+    jd = ax[:, 0] * ae[:, 1] - ax[:, 1] * ae[:, 0]
+    return x, ax, ae, jd
+
+
+class Mesh2D(Box2D):
+    """The same solver on a quadrilateral mesh; lo/hi give the periodic cell
+    of the exact solutions only."""
+
+    def __init__(self, order, basis, flux, mesh, lo, hi, gamma=1.4, R=1.0, ic="IsentropicVortex",
+                 ic_data=(1.0, 1.0, 1.0, 1.0, 5.0)):
+        xyz = np.asarray(mesh[0], dtype=np.float64)
+        e2n = np.asarray(mesh[1], dtype=np.int64)
+        self.ifaces = np.asarray(mesh[2], dtype=np.int64)
+        self.p, self.basis, self.flux = order, basis, flux
+        self.lo, self.hi = np.asarray(lo, float), np.asarray(hi, float)
+        self.gamma, self.R, self.ic, self.ic_data = gamma, R, ic, tuple(ic_data)
+        self.E = len(e2n)
+        self.nb = (order + 1) ** 2
+        self.time = 0.0
+        p = order
+        self.X = xyz[e2n]                                   # [E][4][2]
+        self.vpts, self.vwts = tensor_rule(nq1d(quad_qorder(p)))
+        self.vval, self.vgrad = basis_2d(basis, p, self.vpts)
+        self.mpts, self.mwts = tensor_rule(nq1d(quad_qorder(2 * p)))
+        self.mval, _ = basis_2d(basis, p, self.mpts)
+        # volume metric: cofactor rows C_xi = (a_eta.y, -a_eta.x),
+        # C_eta = (-a_xi.y, a_xi.x), i.e. |J| (J^-1) rows
+        self.cof = np.zeros((self.E, len(self.vpts), 2, 2))
+        self.mx = np.zeros((self.E, len(self.mpts), 2))
+        self.mjd = np.zeros((self.E, len(self.mpts)))
+        self.iMMe = np.zeros((self.E, self.nb, self.nb))
+        for e in range(self.E):
+            _, ax, ae, jd = bilinear(self.X[e], self.vpts)
+            if np.any(jd <= 0):
+                raise ValueError(f"element {e} is folded")
+            self.cof[e, :, 0, 0], self.cof[e, :, 0, 1] = ae[:, 1], -ae[:, 0]
+            self.cof[e, :, 1, 0], self.cof[e, :, 1, 1] = -ax[:, 1], ax[:, 0]
+            x, _, _, jdm = bilinear(self.X[e], self.mpts)
+            self.mx[e], self.mjd[e] = x, jdm
+            MM = np.einsum("qa,qb,q->ab", self.mval, self.mval, self.mwts * jdm)
+            self.iMMe[e] = np.linalg.inv(MM)
+        self.fs, self.fw = O.gl_rule(nq1d(seg_qorder(p)))
+        self.U = np.zeros((self.E, self.nb, NS2))
+
+    def init_state(self):
+        u = ic_state(self.ic, self.ic_data, self.gamma, self.R, self.mx.reshape(-1, 2)).reshape(self.E, -1, NS2)
+        rhs = np.einsum("qb,eqs,eq->ebs", self.mval, u, self.mwts[None, :] * self.mjd)
+        self.U = np.einsum("eab,ebs->eas", self.iMMe, rhs)
+        self.time = 0.0
+
+    def residual(self, U):
+        U = np.asarray(U, dtype=np.float64).reshape(self.E, self.nb, NS2)
+        g = self.gamma
+        Uq = np.einsum("qb,ebs->eqs", self.vval, U)
+        F = flux_interior(Uq.reshape(-1, NS2), g).reshape(self.E, -1, 2, NS2)     # [e][q][j][s]
+        G = np.einsum("eqdj,eqjs->eqds", self.cof, F) * self.vwts[None, :, None, None]
+        R = np.einsum("qbd,eqds->ebs", self.vgrad, G)
+        for eL, fL, oL, eR, fR, oR in self.ifaces:
+            vL, _ = basis_2d(self.basis, self.p, points_on_face(fL, oL, self.fs))
+            vR, _ = basis_2d(self.basis, self.p, points_on_face(fR, oR, self.fs))
+            UL, UR = vL @ U[eL], vR @ U[eR]
+            a, b = FACE_NODES[fL]
+            d = (self.X[eL, b] - self.X[eL, a]) / 2.
+            N = np.tile([d[1], -d[0]], (len(self.fs), 1))
+            Fw = flux_numerical(self.flux, UL, UR, N, g) * self.fw[:, None]
+            R[eL] -= vL.T @ Fw
+            R[eR] += vR.T @ Fw
+        return R
+
+    def inv_mass(self, dt, R):
+        return dt * np.einsum("eab,ebs->eas", self.iMMe, np.asarray(R).reshape(self.E, self.nb, NS2))
+
+    def l2_error(self, t):
+        ue = ic_state(self.ic, self.ic_data, self.gamma, self.R, self.mx.reshape(-1, 2), t).reshape(self.E, -1, NS2)
+        uh = np.einsum("qb,ebs->eqs", self.mval, self.U)
+        w = self.mwts[None, :] * self.mjd
+        return np.sqrt(np.einsum("eqs,eq->s", (uh - ue) ** 2, w) / w.sum())
+
+    def integrals(self):
+        uh = np.einsum("qb,ebs->eqs", self.mval, self.U)
+        return np.einsum("eqs,eq->s", uh, self.mwts[None, :] * self.mjd)

@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 import dg_amd
@@ -190,3 +191,39 @@ def test_valid_2d_cases_reach_the_device():
             "data = [1.0, 1.0, 1.0, 1.0, 5.0]", "data = [" + ", ".join(repr(float(x)) for x in data) + "]")
         rc, msg = _create(text)
         assert rc == want, msg
+
+
+MESH_CASE2 = dg_amd.case_toml(order=2, basis="LagrangeQuad", flux="HLLC", timestep=0.01, lower=(0, 0),
+                              upper=(1, 1), ic="Uniform", ic_data=(1.0, 0.1, 0.2, 2.5), box=False)
+
+
+def test_2d_mesh_input_is_checked_before_the_device():
+    """Quadrilateral meshes (NodeCoords [n][2], Elem2Nodes [E][4], IFaceData
+    with orientations 0..1, src/mesh/mesh.cpp:79-175) get the checks of the
+    3D ones."""
+    from dg_amd import hip
+    import meshgen2d
+    xyz, e2n, ifc = meshgen2d.box_mesh2d((3, 2), (0, 0), (1, 1), warp=0.1, seed=4, rotate=True, shuffle=True,
+                                         random_left=True)
+    rc, msg = _create_mesh(MESH_CASE2, xyz, e2n, ifc)
+    assert rc == (0 if hip.device_count() > 0 else 2), msg
+    bad = ifc.copy()
+    bad[2, 5] = 2                                   # orientation outside 0..1
+    rc, msg = _create_mesh(MESH_CASE2, xyz, e2n, bad)
+    assert rc == 1 and "not 0..1" in msg, msg
+    bad = ifc.copy()
+    bad[0, 4] = 4                                   # face id outside 0..3
+    rc, msg = _create_mesh(MESH_CASE2, xyz, e2n, bad)
+    assert rc == 1 and "out of range" in msg, msg
+    rc, msg = _create_mesh(MESH_CASE2, xyz, e2n, ifc[1:])
+    assert rc == 4 and "boundary faces" in msg, msg
+    bad = ifc.copy()
+    bad[1] = bad[0]
+    rc, msg = _create_mesh(MESH_CASE2, xyz, e2n, bad)
+    assert rc == 1 and "twice" in msg, msg
+    e = e2n.copy()
+    e[1, 3] = -1
+    rc, msg = _create_mesh(MESH_CASE2, xyz, e, ifc)
+    assert rc == 1 and "node index" in msg, msg
+    with pytest.raises(ValueError, match="2D"):
+        dg_amd.Solver(MESH_CASE2, mesh=(xyz, np.zeros((6, 8), dtype=np.int64), ifc))

@@ -1,8 +1,8 @@
-"""Hex mesh files on the CPU: the datasets the reference's mesh reader opens
+"""Mesh files on the CPU: the datasets the reference's mesh reader opens
 (Mesh::read_mesh, src/mesh/mesh.cpp:17-26,79-200), read from the
 reference's own mesh file (examples/quad_2x2/quad_2x2.h5, a 2D quad mesh,
-kept as tests/golden/quad_2x2.h5) and written / read back for 3D hex
-meshes.  Values are checked against the HDF5 command-line dump (h5dump),
+kept as tests/golden/quad_2x2.h5) and written / read back for 3D hex and
+2D quadrilateral meshes.  Values are checked against the HDF5 command-line dump (h5dump),
 independently of the library."""
 import ctypes as C
 import os
@@ -65,10 +65,46 @@ def test_reads_the_reference_mesh_file():
         assert np.array_equal(_dump(QUAD, "IFaceData"), ifc.ravel())
 
 
-def test_quad_mesh_is_not_on_the_hex_path():
+CASE2 = dg_amd.case_toml(order=2, basis="LagrangeQuad", flux="HLLC", timestep=0.01, lower=(0, 0), upper=(1, 1),
+                         ic="Uniform", ic_data=(1.0, 0.1, 0.2, 2.5), box=False)
+
+
+def test_reference_quad_mesh_file_checks():
+    """The file's Dimension must be the case's; the reference's 2x2 quad
+    mesh has boundary face groups (walls), which this periodic path
+    reports as unsupported."""
     with pytest.raises(dg_amd.DGError) as ei:
         h5.solver_from_mesh_file(CASE, QUAD)
-    assert ei.value.status == 4 and "Dimension = 2" in str(ei.value)
+    assert ei.value.status == 1 and "Dimension = 2 does not match Physics.dim = 3" in str(ei.value)
+    with pytest.raises(dg_amd.DGError) as ei:
+        h5.solver_from_mesh_file(CASE2, QUAD)
+    assert ei.value.status == 4 and "boundary face groups" in str(ei.value)
+
+
+def test_write_read_round_trip_2d(tmp_path):
+    import meshgen2d
+    from dg_amd import hip
+    xyz, e2n, ifc = meshgen2d.box_mesh2d((3, 4), (0, 0), (1, 1), warp=0.1, seed=2, rotate=True, shuffle=True,
+                                         random_left=True)
+    f = str(tmp_path / "quad.h5")
+    h5.write_mesh(f, xyz, e2n, ifc)
+    info = h5.read_mesh_info(f)
+    assert info == {"dim": 2, "qorder": 1, "num_elems": 12, "num_nodes": 4 * 5, "num_ifaces": 24,
+                    "num_nodes_per_elem": 4, "num_nodes_per_face": 2, "num_bface_groups": 0}
+    x2, e2, i2 = h5.read_mesh(f)
+    assert np.array_equal(x2, xyz) and np.array_equal(e2, e2n) and np.array_equal(i2, ifc)
+    if H5DUMP:
+        ref, mine = _types(QUAD), _types(f)
+        for name, t in mine.items():
+            assert ref[name] == t, name
+        assert set(mine) == {k for k in ref if not k.startswith("BFG_")}
+        assert np.array_equal(_dump(f, "NodeCoords"), xyz.ravel())
+        assert np.array_equal(_dump(f, "Elem2Nodes"), e2n.ravel())
+    h = C.c_void_p()
+    rc = h5.lib().dg_create_from_mesh_file(CASE2.encode(), f.encode(), C.byref(h))
+    if rc == 0:
+        dg_amd.lib().dg_destroy(h)
+    assert rc == (0 if hip.device_count() > 0 else 2), h5.lib().dg_h5_last_error().decode()
 
 
 def test_write_read_round_trip(tmp_path):
