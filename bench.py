@@ -12,6 +12,10 @@ the stream the stage kernels are launched on.
 
 value = DOF-updates per second per RK stage = E (p+1)^3 / t_stage.
 
+Beside it, "fp32_form": the same run with Numerics.precision = "FP32" (face
+and volume work of the stage kernel in FP32, RK registers, M^-1 and the L2
+reductions in FP64): its rate, and its L2 error against the FP64 run's.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 import argparse
@@ -40,7 +44,7 @@ def alg_flops_per_element(p):
     return 2 * (4 * V * S + 12 * V * T) + 60 * m ** 3 + 300 * m ** 2 + 4 * V * n ** 3
 
 
-def case_toml(ne, p=P, end=1.0, warp=0.0, dim=3):
+def case_toml(ne, p=P, end=1.0, warp=0.0, dim=3, precision=None):
     import dg_amd
     if dim == 2:
         # the 2D path: ne x ne quadrilaterals, the reference's 2D vortex
@@ -49,7 +53,8 @@ def case_toml(ne, p=P, end=1.0, warp=0.0, dim=3):
                                 ic_data=(1.0, 1.0, 1.0, 1.0, 5.0))
     return dg_amd.case_toml(order=p, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne),
                             lower=(-5., -5., -5.), upper=(5., 5., 5.), stepper="RK4", cfl=0.1, end=end,
-                            ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0), warp=warp)
+                            ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0), warp=warp,
+                            precision=precision)
 
 
 def measured_traffic():
@@ -122,6 +127,9 @@ def main():
     # data.h5 write / read-back of the workload's state (file path)
     ap.add_argument("--io", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    # the FP32 form only (for profiling its kernel); --no-fp32 skips it
+    ap.add_argument("--fp32", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -136,31 +144,36 @@ def main():
     ne, order, dim = args.ne, args.order, args.dim
     if dim == 2 and ne == NE:
         ne = 1024                         # 2D: 1024^2 elements, the DOF count of the 3D headline at p=3
-    s = dg_amd.Solver(case_toml(ne, order, warp=args.warp, dim=dim))
     stream = hip.Stream()                 # default priority
-    s.set_stream(stream.handle)
-    s.init_state_from_fcn()
-    dt = s.timestep
     E = ne ** dim
     nb = (order + 1) ** dim
     nvar = dim + 2
     dof = E * nb
 
+    def timed_run(precision):
+        """Warm-up and timed RK4 steps; (solver, ms per step, wall s)."""
+        s = dg_amd.Solver(case_toml(ne, order, warp=args.warp, dim=dim, precision=precision))
+        s.set_stream(stream.handle)
+        s.init_state_from_fcn()
+        dt = s.timestep
+        s.take_time_step(dt, args.warmup)
+        hip.synchronize()
+        ev0, ev1 = hip.Event(), hip.Event()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        s.take_time_step(dt, args.steps)      # 4 stage launches per step, all on `stream`
+        ev1.record(stream)
+        ev1.synchronize()
+        hip.synchronize()
+        return s, ev0.elapsed_ms(ev1) / args.steps, time.perf_counter() - t0
+
     if args.io:
+        s = dg_amd.Solver(case_toml(ne, order, warp=args.warp, dim=dim))
+        s.init_state_from_fcn()
         io_bench(s, args.io)
         return
-    s.take_time_step(dt, args.warmup)
-    hip.synchronize()
-    ev0, ev1 = hip.Event(), hip.Event()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    s.take_time_step(dt, args.steps)      # 4 stage launches per step, all on `stream`
-    ev1.record(stream)
-    ev1.synchronize()
-    hip.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_ms(ev1)
-    ms_per_step = ms / args.steps
+    s, ms_per_step, wall = timed_run("FP32" if args.fp32 else None)
+    dt = s.timestep
     t_stage = ms_per_step / 4 * 1e-3
     value = dof / t_stage
 
@@ -182,7 +195,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.fp32 else "f64",
         "data": "synthetic (z-extruded isentropic vortex, generated on the GPU)" if dim == 3 else
                 "synthetic (2D isentropic vortex, generated on the GPU)",
         "config": {"workload": (f"euler3d_periodic_box_{ne}^3_p{order}_legendre_roe_rk4_cfl0.1"
@@ -195,7 +208,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if (order == P and ne == NE and not args.warp and dim == 3)
                      else None},
-        "kernel": {"name": f"stage_kernel<{order},Roe{',mapped' if args.warp else ''}>" if dim == 3 else
+        "kernel": {"name": f"stage_kernel<{order},Roe{',mapped' if args.warp else ''}{',fp32' if args.fp32 else ''}>"
+                   if dim == 3 else
                    f"stage2d_kernel<{order},Roe>",
                    "avg_launch_ms": ms_per_step / 4,
                    "launches": 4 * args.steps, "alg_bytes_per_launch": bytes_per_launch,
@@ -206,7 +220,27 @@ def main():
         "finite": finite,
         "wall_s": wall,
     }
-    s.close()
+    if dim == 3 and not args.warp and not args.fp32 and not args.no_fp32:
+        # the FP32 form on the same workload, against this FP64 run
+        U64 = s.get_state()
+        s.close()
+        s32, ms32, _ = timed_run("FP32")
+        err32 = s32.get_error()
+        U32 = s32.get_state()
+        diff = float(np.max(np.abs(U32 - U64)) / np.max(np.abs(U64)))
+        del U32, U64
+        out["fp32_form"] = {
+            "value": dof / (ms32 / 4 * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms32,
+            "speedup_vs_fp64": ms_per_step / ms32,
+            "dtype": "f32 traces/fluxes/contractions, f64 RK registers, M^-1 and reductions",
+            "kernel": f"stage_kernel<{order},Roe,fp32>",
+            "l2_error_vs_exact": [float(x) for x in err32],
+            "l2_error_ratio_vs_fp64": [float(a / b) if b > 0 else None for a, b in zip(err32, err)],
+            "state_max_rel_diff_vs_fp64": diff,
+        }
+        s32.close()
+    else:
+        s.close()
     stream.destroy()
     if not args.no_cpu and dim == 3:
         out["cpu_baseline"] = cpu_baseline()

@@ -266,6 +266,8 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     int qi = pick(quad, {"GaussLegendre", "GaussLobatto", "TriangleDunavant", "TriangleGaussLegendre"});
     if (qi != 0) throw UnsupportedError("Numerics.quad = " + quad + " is not implemented");
     if (c.order < 1) throw UnsupportedError("Numerics.order must be >= 1 on this path");
+    c.fp32 = pick(get_str(d, "Numerics", "precision", false, "FP64"), {"FP64", "FP32"}) == 1;
+    if (c.fp32 && dim != 3) throw UnsupportedError("Numerics.precision = FP32 with Physics.dim = 2 (3D box only)");
     if (c.basis == Basis::LegendreHex && c.order > 7)
         throw InputError("Legendre polynomial P = " + std::to_string(c.order) + " not implemented.");
 
@@ -327,6 +329,7 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
         else if (nrm != "reference") throw InputError("Mesh.normals must be \"reference\" or \"exact\"");
         if (get_num(d, "Mesh", "warp", false, 0.0) != 0.0)
             throw InputError("Mesh.warp applies to the periodic box only");
+        if (c.fp32) throw UnsupportedError("Numerics.precision = FP32 on a mesh (the periodic box only)");
         if (c.timestep <= 0.0) throw InputError("Stepper.timestep is required with a mesh (Stepper.cfl needs the box)");
         if (dim == 3 && c.order > 4)
             throw UnsupportedError("meshes with order > 4: the per-element mass matrix inverse of the "
@@ -379,6 +382,7 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     if (c.warp > 0.0 && c.order > 4)
         throw UnsupportedError("Mesh.warp with order > 4: the per-element mass matrix inverse of the "
                                "non-affine path is formed in LDS and holds at most (4+1)^3 basis functions");
+    if (c.fp32 && c.warp > 0.0) throw UnsupportedError("Numerics.precision = FP32 with Mesh.warp (affine box only)");
     return c;
 }
 

@@ -64,13 +64,44 @@ struct Dims {
 // indices, copied to LDS once per workgroup.  Reading them from global
 // memory would issue vector loads, and waiting for those would also wait
 // for every prefetch load issued before them.
-struct LaneTab {
-    double w[PMAX + 2];      // GL weights of the (p+2)-point rule
-    double phim[PMAX + 1];   // phi_i(-1)
-    double phip[PMAX + 1];   // phi_i(+1)
+// R: the compute type of the stage kernel (the M^-1 diagonal stays FP64).
+template <class R>
+struct LaneTabT {
+    R w[PMAX + 2];           // GL weights of the (p+2)-point rule
+    R phim[PMAX + 1];        // phi_i(-1)
+    R phip[PMAX + 1];        // phi_i(+1)
     double mdiag[PMAX + 1];  // diagonal of the 1D inverse mass matrix
-    double xv[PMAX + 2];     // GL nodes of the (p+2)-point rule
+    R xv[PMAX + 2];          // GL nodes of the (p+2)-point rule
 };
+using LaneTab = LaneTabT<double>;
+
+// Reference-element tables in the compute type R of the stage kernel
+// (FP32: the rounded copies the host stores beside the FP64 tables).
+template <class R>
+__device__ __forceinline__ R tB(const KTables& T, int a, int i) {
+    if constexpr (sizeof(R) == 4) return T.B32[a][i];
+    else return T.B[a][i];
+}
+template <class R>
+__device__ __forceinline__ R tD(const KTables& T, int a, int i) {
+    if constexpr (sizeof(R) == 4) return T.D32[a][i];
+    else return T.D[a][i];
+}
+template <class R>
+__device__ __forceinline__ R tw(const KTables& T, int a) {
+    if constexpr (sizeof(R) == 4) return T.w32[a];
+    else return T.w[a];
+}
+template <class R>
+__device__ __forceinline__ R tphim(const KTables& T, int i) {
+    if constexpr (sizeof(R) == 4) return T.phim32[i];
+    else return T.phim[i];
+}
+template <class R>
+__device__ __forceinline__ R tphip(const KTables& T, int i) {
+    if constexpr (sizeof(R) == 4) return T.phip32[i];
+    else return T.phip[i];
+}
 
 template <int T, int TS = WAVE, class F>
 __device__ __forceinline__ void team_for(int lane, F&& f) {
@@ -96,8 +127,9 @@ __device__ __forceinline__ double pressure(double gam, const double* U) {
 }
 
 // Pressure with 1/rho supplied (one reciprocal per state, reused below).
-__device__ __forceinline__ double pressure_r(double gam, const double* U, double r1) {
-    return (gam - 1.0) * (U[4] - 0.5 * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * r1));
+template <class R>
+__device__ __forceinline__ R pressure_r(R gam, const R* U, R r1) {
+    return (gam - R(1)) * (U[4] - R(0.5) * ((U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * r1));
 }
 
 // Reciprocal and square root of the flux computations: hardware estimates
@@ -129,6 +161,15 @@ __device__ __forceinline__ double fsqrt(double x) {
     return x > 0.0 ? g : (x == 0.0 ? 0.0 : __builtin_nan(""));
 }
 
+// Single precision: the hardware reciprocal refined by one Newton-Raphson
+// step, and the hardware square root (both within about an ulp).
+__device__ __forceinline__ float frcp(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // src/physics/euler/functions.cpp:25-40
 __device__ __forceinline__ double max_wave_speed(double gam, const double* U) {
     const double rho1 = 1. / U[0];
@@ -138,101 +179,99 @@ __device__ __forceinline__ double max_wave_speed(double gam, const double* U) {
 
 // Euler flux along axis D (one block of src/physics/euler/functions.cpp:64-91)
 // with 1/rho and p supplied.
-template <int D>
-__device__ __forceinline__ void flux_axis(const double* U, double r1, double P, double* F) {
-    const double un = U[1 + D];
+template <int D, class R>
+__device__ __forceinline__ void flux_axis(const R* U, R r1, R P, R* F) {
+    const R un = U[1 + D];
     F[0] = un;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const double t = un * U[1 + i] * r1;
+        const R t = un * U[1 + i] * r1;
         F[1 + i] = (i == D) ? t + P : t;
     }
     F[4] = (U[4] + P) * un * r1;
 }
 
 // Local Lax-Friedrichs with N = nmag e_D (src/physics/base/functions.cpp:33-89).
-template <int D>
-__device__ __forceinline__ void riemann_llf(double gam, const double* UL, const double* UR,
-                                            double nmag, double* F) {
-    const double r1L = frcp(UL[0]), r1R = frcp(UR[0]);
-    const double PL = pressure_r(gam, UL, r1L), PR = pressure_r(gam, UR, r1R);
-    double FL[V], FR[V];
+template <int D, class R>
+__device__ __forceinline__ void riemann_llf(R gam, const R* UL, const R* UR, R nmag, R* F) {
+    const R r1L = frcp(UL[0]), r1R = frcp(UR[0]);
+    const R PL = pressure_r(gam, UL, r1L), PR = pressure_r(gam, UR, r1R);
+    R FL[V], FR[V];
     flux_axis<D>(UL, r1L, PL, FL);
     flux_axis<D>(UR, r1R, PR, FR);
-    double a = fsqrt(UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) * r1L + fsqrt(gam * PL * r1L);
-    const double aR = fsqrt(UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]) * r1R + fsqrt(gam * PR * r1R);
+    R a = fsqrt(UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) * r1L + fsqrt(gam * PL * r1L);
+    const R aR = fsqrt(UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]) * r1R + fsqrt(gam * PR * r1R);
     if (aR > a) a = aR;
 #pragma unroll
-    for (int s = 0; s < V; ++s) F[s] = nmag * (0.5 * (FL[s] + FR[s]) - 0.5 * a * (UR[s] - UL[s]));
+    for (int s = 0; s < V; ++s) F[s] = nmag * (R(0.5) * (FL[s] + FR[s]) - R(0.5) * a * (UR[s] - UL[s]));
 }
 
 // HLLC with PVRS wave speeds, N = nmag e_D (src/physics/euler/functions.cpp:298-399).
-template <int D>
-__device__ __forceinline__ void riemann_hllc(double gam, const double* UL, const double* UR,
-                                             double nmag, double* F) {
-    const double djac = nmag;
-    const double rhoL = UL[0], rhoL1 = frcp(rhoL);
-    const double unL = UL[1 + D] * rhoL1;
-    const double PL = pressure_r(gam, UL, rhoL1);
-    const double aL = fsqrt(gam * PL * rhoL1);
-    const double rhoR = UR[0], rhoR1 = frcp(rhoR);
-    const double unR = UR[1 + D] * rhoR1;
-    const double PR = pressure_r(gam, UR, rhoR1);
-    const double aR = fsqrt(gam * PR * rhoR1);
-    const double rAvg = 0.5 * (rhoL + rhoR);
-    const double aAvg = 0.5 * (aL + aR);
-    const double pStar = fmax(0., 0.5 * (PL + PR - (unR - unL) * rAvg * aAvg));
-    const double gq = (gam + 1.) / (2. * gam);
-    double qL = 1.0;
-    const double xL = pStar * frcp(PL);
-    if (xL > 1.) qL = fsqrt(1. + gq * (xL - 1.));
-    double qR = 1.0;
-    const double xR = pStar * frcp(PR);
-    if (xR > 1.) qR = fsqrt(1. + gq * (xR - 1.));
-    const double sL = unL - aL * qL;
-    const double sR = unR + aR * qR;
-    const double sStar = (PR - PL + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR)) *
+template <int D, class R>
+__device__ __forceinline__ void riemann_hllc(R gam, const R* UL, const R* UR, R nmag, R* F) {
+    const R djac = nmag;
+    const R rhoL = UL[0], rhoL1 = frcp(rhoL);
+    const R unL = UL[1 + D] * rhoL1;
+    const R PL = pressure_r(gam, UL, rhoL1);
+    const R aL = fsqrt(gam * PL * rhoL1);
+    const R rhoR = UR[0], rhoR1 = frcp(rhoR);
+    const R unR = UR[1 + D] * rhoR1;
+    const R PR = pressure_r(gam, UR, rhoR1);
+    const R aR = fsqrt(gam * PR * rhoR1);
+    const R rAvg = R(0.5) * (rhoL + rhoR);
+    const R aAvg = R(0.5) * (aL + aR);
+    const R pStar = fmax(R(0.), R(0.5) * (PL + PR - (unR - unL) * rAvg * aAvg));
+    const R gq = (gam + R(1.)) / (R(2.) * gam);
+    R qL = R(1.0);
+    const R xL = pStar * frcp(PL);
+    if (xL > R(1.)) qL = fsqrt(R(1.) + gq * (xL - R(1.)));
+    R qR = R(1.0);
+    const R xR = pStar * frcp(PR);
+    if (xR > R(1.)) qR = fsqrt(R(1.) + gq * (xR - R(1.)));
+    const R sL = unL - aL * qL;
+    const R sR = unR + aR * qR;
+    const R sStar = (PR - PL + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR)) *
                          frcp(rhoL * (sL - unL) - rhoR * (sR - unR));
-    if (sL >= 0.) {
+    if (sL >= R(0.)) {
         F[0] = djac * rhoL * unL;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double t = UL[1 + i] * unL;
+            const R t = UL[1 + i] * unL;
             F[1 + i] = djac * ((i == D) ? t + PL : t);
         }
         F[4] = djac * ((UL[4] + PL) * unL);
-    } else if (sR <= 0.) {
+    } else if (sR <= R(0.)) {
         F[0] = djac * rhoR * unR;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double t = UR[1 + i] * unR;
+            const R t = UR[1 + i] * unR;
             F[1 + i] = djac * ((i == D) ? t + PR : t);
         }
         F[4] = djac * (UR[4] + PR) * unR;
-    } else if ((sL < 0.) && (sStar >= 0.)) {
-        const double c = (sL - unL) * frcp(sL - sStar);
-        const double f1 = sStar - unL;
-        const double f2 = sStar + PL * frcp(rhoL * (sL - unL));
-        F[0] = djac * (rhoL * unL + rhoL * sL * (c - 1.));
+    } else if ((sL < R(0.)) && (sStar >= R(0.))) {
+        const R c = (sL - unL) * frcp(sL - sStar);
+        const R f1 = sStar - unL;
+        const R f2 = sStar + PL * frcp(rhoL * (sL - unL));
+        F[0] = djac * (rhoL * unL + rhoL * sL * (c - R(1.)));
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double t = UL[1 + i] * unL;
-            const double u = UL[1 + i] * (c - 1.);
+            const R t = UL[1 + i] * unL;
+            const R u = UL[1 + i] * (c - R(1.));
             F[1 + i] = djac * ((i == D) ? (t + PL + sL * (u + rhoL * c * f1)) : (t + sL * u));
         }
-        F[4] = djac * ((UL[4] + PL) * unL + sL * (UL[4] * (c - 1.) + rhoL * c * f1 * f2));
-    } else if ((sStar < 0.) && (sR > 0.)) {
-        const double c = (sR - unR) * frcp(sR - sStar);
-        const double f1 = sStar - unR;
-        const double f2 = sStar + PR * frcp(rhoR * (sR - unR));
-        F[0] = djac * (rhoR * unR + rhoR * sR * (c - 1.));
+        F[4] = djac * ((UL[4] + PL) * unL + sL * (UL[4] * (c - R(1.)) + rhoL * c * f1 * f2));
+    } else if ((sStar < R(0.)) && (sR > R(0.))) {
+        const R c = (sR - unR) * frcp(sR - sStar);
+        const R f1 = sStar - unR;
+        const R f2 = sStar + PR * frcp(rhoR * (sR - unR));
+        F[0] = djac * (rhoR * unR + rhoR * sR * (c - R(1.)));
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double t = UR[1 + i] * unR;
-            const double u = UR[1 + i] * (c - 1.);
+            const R t = UR[1 + i] * unR;
+            const R u = UR[1 + i] * (c - R(1.));
             F[1 + i] = djac * ((i == D) ? (t + PR + sR * (u + rhoR * c * f1)) : (t + sR * u));
         }
-        F[4] = djac * ((UR[4] + PR) * unR + sR * (UR[4] * (c - 1.) + rhoR * c * f1 * f2));
+        F[4] = djac * ((UR[4] + PR) * unR + sR * (UR[4] * (c - R(1.)) + rhoR * c * f1 * f2));
     } else {
         // the reference prints and asserts (functions.cpp:389-398); NaN
         // propagates to the host-side finiteness check instead
@@ -244,69 +283,68 @@ __device__ __forceinline__ void riemann_hllc(double gam, const double* UL, const
 // Roe-Pike flux with a Harten-Hyman-type entropy fix on the acoustic waves
 // (no reference counterpart; the algorithm of the oracle's flux_roe,
 // arranged around four reciprocals and five square roots).
-__device__ __forceinline__ double entropy_fix(double lam, double lamL, double lamR) {
-    const double delta = fmax(0., fmax(lam - lamL, lamR - lam));
-    double a = fabs(lam);
-    if (a < delta) a = 0.5 * (lam * lam + delta * delta) / delta;   // rare: transonic expansion
+template <class R>
+__device__ __forceinline__ R entropy_fix(R lam, R lamL, R lamR) {
+    const R delta = fmax(R(0), fmax(lam - lamL, lamR - lam));
+    R a = fabs(lam);
+    if (a < delta) a = R(0.5) * (lam * lam + delta * delta) / delta;   // rare: transonic expansion
     return a;
 }
 
-template <int D>
-__device__ __forceinline__ void riemann_roe(double gam, const double* UL, const double* UR,
-                                            double nmag, double* F) {
-    const double rL = UL[0], rR = UR[0];
-    const double r1L = frcp(rL), r1R = frcp(rR);
-    double uL[3], uR[3];
+template <int D, class R>
+__device__ __forceinline__ void riemann_roe(R gam, const R* UL, const R* UR, R nmag, R* F) {
+    const R rL = UL[0], rR = UR[0];
+    const R r1L = frcp(rL), r1R = frcp(rR);
+    R uL[3], uR[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         uL[d] = UL[1 + d] * r1L;
         uR[d] = UR[1 + d] * r1R;
     }
-    const double gm1 = gam - 1.;
-    const double pL = gm1 * (UL[4] - 0.5 * (UL[1] * uL[0] + UL[2] * uL[1] + UL[3] * uL[2]));
-    const double pR = gm1 * (UR[4] - 0.5 * (UR[1] * uR[0] + UR[2] * uR[1] + UR[3] * uR[2]));
-    const double HL = (UL[4] + pL) * r1L, HR = (UR[4] + pR) * r1R;
-    const double sL = fsqrt(rL), sR = fsqrt(rR), isum = frcp(sL + sR);
-    double ut[3];
+    const R gm1 = gam - R(1.);
+    const R pL = gm1 * (UL[4] - R(0.5) * (UL[1] * uL[0] + UL[2] * uL[1] + UL[3] * uL[2]));
+    const R pR = gm1 * (UR[4] - R(0.5) * (UR[1] * uR[0] + UR[2] * uR[1] + UR[3] * uR[2]));
+    const R HL = (UL[4] + pL) * r1L, HR = (UR[4] + pR) * r1R;
+    const R sL = fsqrt(rL), sR = fsqrt(rR), isum = frcp(sL + sR);
+    R ut[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) ut[d] = (sL * uL[d] + sR * uR[d]) * isum;
-    const double Ht = (sL * HL + sR * HR) * isum;
-    const double q2 = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
-    const double c2 = gm1 * (Ht - 0.5 * q2);
-    const double ct = fsqrt(c2);
-    const double ic2 = frcp(c2);
-    const double rt = sL * sR;
-    const double unt = ut[D], unL = uL[D], unR = uR[D];
-    const double drho = rR - rL, dp = pR - pL, dun = unR - unL;
-    const double a1 = (dp - rt * ct * dun) * (0.5 * ic2);
-    const double a5 = (dp + rt * ct * dun) * (0.5 * ic2);
-    const double a2 = drho - dp * ic2;
-    const double cL = fsqrt(gam * pL * r1L), cR = fsqrt(gam * pR * r1R);
-    const double l1 = entropy_fix(unt - ct, unL - cL, unR - cR);
-    const double l2 = fabs(unt);
-    const double l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
-    const double w1 = l1 * a1, w5 = l5 * a5, w2 = l2 * a2, ws = l2 * rt;
-    double Dv[V];
+    const R Ht = (sL * HL + sR * HR) * isum;
+    const R q2 = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
+    const R c2 = gm1 * (Ht - R(0.5) * q2);
+    const R ct = fsqrt(c2);
+    const R ic2 = frcp(c2);
+    const R rt = sL * sR;
+    const R unt = ut[D], unL = uL[D], unR = uR[D];
+    const R drho = rR - rL, dp = pR - pL, dun = unR - unL;
+    const R a1 = (dp - rt * ct * dun) * (R(0.5) * ic2);
+    const R a5 = (dp + rt * ct * dun) * (R(0.5) * ic2);
+    const R a2 = drho - dp * ic2;
+    const R cL = fsqrt(gam * pL * r1L), cR = fsqrt(gam * pR * r1R);
+    const R l1 = entropy_fix(unt - ct, unL - cL, unR - cR);
+    const R l2 = fabs(unt);
+    const R l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
+    const R w1 = l1 * a1, w5 = l5 * a5, w2 = l2 * a2, ws = l2 * rt;
+    R Dv[V];
     Dv[0] = w1 + w5 + w2;
-    double udu = 0.;
+    R udu = R(0.);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double cn = (d == D) ? ct : 0.0;
-        const double dut = (d == D) ? (uR[d] - uL[d]) - dun : (uR[d] - uL[d]);
+        const R cn = (d == D) ? ct : R(0.0);
+        const R dut = (d == D) ? (uR[d] - uL[d]) - dun : (uR[d] - uL[d]);
         Dv[1 + d] = w1 * (ut[d] - cn) + w5 * (ut[d] + cn) + w2 * ut[d] + ws * dut;
         udu += ut[d] * dut;
     }
-    Dv[4] = w1 * (Ht - unt * ct) + w5 * (Ht + unt * ct) + w2 * 0.5 * q2 + ws * udu;
-    double FL[V], FR[V];
+    Dv[4] = w1 * (Ht - unt * ct) + w5 * (Ht + unt * ct) + w2 * R(0.5) * q2 + ws * udu;
+    R FL[V], FR[V];
     flux_axis<D>(UL, r1L, pL, FL);
     flux_axis<D>(UR, r1R, pR, FR);
 #pragma unroll
-    for (int s = 0; s < V; ++s) F[s] = nmag * (0.5 * (FL[s] + FR[s]) - 0.5 * Dv[s]);
+    for (int s = 0; s < V; ++s) F[s] = nmag * (R(0.5) * (FL[s] + FR[s]) - R(0.5) * Dv[s]);
 }
 
-template <int FLUX, int D>
-__device__ __forceinline__ void riemann(double gam, const double* UL, const double* UR, double nmag,
-                                        double* F) {
+template <int FLUX, int D, class R>
+__device__ __forceinline__ void riemann(R gam, const R* UL, const R* UR, R nmag, R* F) {
     if constexpr (FLUX == 0) riemann_llf<D>(gam, UL, UR, nmag, F);
     else if constexpr (FLUX == 1) riemann_hllc<D>(gam, UL, UR, nmag, F);
     else riemann_roe<D>(gam, UL, UR, nmag, F);
@@ -737,29 +775,29 @@ __device__ __forceinline__ void face_normal_gen(const double* X, int d, int side
 // arithmetic of the neighbour's own slab, so both sides of a face see the
 // same trace and the same flux.
 // ---------------------------------------------------------------------------
-template <int P, int FLUX, int TS, bool MAPPED = false>
-__device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, const KGeom& G, int lane,
-                                           const double* sU, double* sR, double* bankA, double* bankB,
+template <int P, int FLUX, int TS, bool MAPPED = false, class R = double>
+__device__ __forceinline__ void face_phase(const KTables& T, const LaneTabT<R>& LT, const KGeom& G, int lane,
+                                           const R* sU, R* sR, R* bankA, R* bankB,
                                            const double* XL = nullptr, unsigned eid = 0) {
     using Dm = Dims<P>;
     using L = StageLDS<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, NNN = Dm::NNN;
-    double* slab = bankB;
-    double* f1 = bankA + L::U;
-    double* f2 = bankB;
-    double* fh = bankA + L::U;
-    double* l1 = bankB;
-    double* l2 = bankA + L::U;
+    R* slab = bankB;
+    R* f1 = bankA + L::U;
+    R* f2 = bankB;
+    R* fh = bankA + L::U;
+    R* l1 = bankB;
+    R* l2 = bankA + L::U;
 
     // 1. own slabs of both sides (the neighbour slabs are already in `slab`)
     team_for<3 * NN * V, TS>(lane, [&](int t) {
         const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, d = u2 / N;
-        double um = 0., up = 0.;
+        R um = 0., up = 0.;
 #pragma unroll
         for (int r = 0; r < N; ++r) {
-            const double x = sU[bidx_d<N>(d, r, t1, t2) * V + s];
-            um = fma(T.phim[r], x, um);
-            up = fma(T.phip[r], x, up);
+            const R x = sU[bidx_d<N>(d, r, t1, t2) * V + s];
+            um = fma(tphim<R>(T, r), x, um);
+            up = fma(tphip<R>(T, r), x, up);
         }
         slab[((d * 4 + 0) * NN + t2 * N + t1) * V + s] = um;
         slab[((d * 4 + 1) * NN + t2 * N + t1) * V + s] = up;
@@ -768,14 +806,14 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
     // 2. interpolate along t1 to the face quadrature points
     team_for<12 * N * V, TS>(lane, [&](int t) {
         const int s = t % V, u = t / V, t2 = u % N, ds = u / N;
-        double x[N];
+        R x[N];
 #pragma unroll
         for (int t1 = 0; t1 < N; ++t1) x[t1] = slab[(ds * NN + t2 * N + t1) * V + s];
 #pragma unroll
         for (int a = 0; a < M; ++a) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
-            for (int t1 = 0; t1 < N; ++t1) acc = fma(T.B[a][t1], x[t1], acc);
+            for (int t1 = 0; t1 < N; ++t1) acc = fma(tB<R>(T, a, t1), x[t1], acc);
             f1[((ds * N + t2) * M + a) * V + s] = acc;
         }
     });
@@ -783,14 +821,14 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
     // 3. interpolate along t2
     team_for<12 * M * V, TS>(lane, [&](int t) {
         const int s = t % V, u = t / V, a1 = u % M, ds = u / M;
-        double x[N];
+        R x[N];
 #pragma unroll
         for (int t2 = 0; t2 < N; ++t2) x[t2] = f1[((ds * N + t2) * M + a1) * V + s];
 #pragma unroll
         for (int a2 = 0; a2 < M; ++a2) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
-            for (int t2 = 0; t2 < N; ++t2) acc = fma(T.B[a2][t2], x[t2], acc);
+            for (int t2 = 0; t2 < N; ++t2) acc = fma(tB<R>(T, a2, t2), x[t2], acc);
             f2[((ds * M + a2) * M + a1) * V + s] = acc;
         }
     });
@@ -802,15 +840,15 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
         const int d = t / L::RP, q = t % L::RP;
         if (q >= 2 * MM) return;
         const int a1 = q % M, u = q / M, a2 = u % M, side = u / M;
-        const double* own = f2 + (((d * 4 + side) * M + a2) * M + a1) * V;
-        const double* nbv = f2 + (((d * 4 + 2 + side) * M + a2) * M + a1) * V;
-        double UL[V], UR[V], F[V];
+        const R* own = f2 + (((d * 4 + side) * M + a2) * M + a1) * V;
+        const R* nbv = f2 + (((d * 4 + 2 + side) * M + a2) * M + a1) * V;
+        R UL[V], UR[V], F[V];
 #pragma unroll
         for (int s = 0; s < V; ++s) {
             UL[s] = side == 0 ? nbv[s] : own[s];
             UR[s] = side == 0 ? own[s] : nbv[s];
         }
-        double sgn = side == 0 ? 1. : -1.;   // the box: the + face's element is the left one
+        R sgn = side == 0 ? 1. : -1.;   // the box: the + face's element is the left one
         if constexpr (MAPPED) {
             // general connectivity: the neighbour's trace sits at its own
             // face point (b1, b2); the flux is evaluated with the face's
@@ -821,7 +859,7 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
             const bool isL = (meta >> 6) & 1;
             int b1, b2;
             map_point((meta >> 3) & 7, M, a1, a2, b1, b2);
-            const double* nbp = f2 + (((d * 4 + 2 + side) * M + b2) * M + b1) * V;
+            const R* nbp = f2 + (((d * 4 + 2 + side) * M + b2) * M + b1) * V;
 #pragma unroll
             for (int s = 0; s < V; ++s) {
                 UL[s] = isL ? own[s] : nbp[s];
@@ -829,19 +867,19 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
             }
             const int lfo = (meta >> 8) & 63, fL = lfo >> 3;
             const int c1 = isL ? a1 : b1, c2 = isL ? a2 : b2;
-            const double* ev = G.nev + ((size_t)lfo * MM + c2 * M + c1) * 2;
-            double Nv[3];
+            const R* ev = G.nev + ((size_t)lfo * MM + c2 * M + c1) * 2;
+            R Nv[3];
             face_normal_gen(XL + (isL ? 0 : (1 + d * 2 + side) * 24), face_dir(fL), face_side(fL), ev[0], ev[1],
                             Nv);
-            riemann_n<FLUX>(G.gamma, UL, UR, Nv, F);
+            riemann_n<FLUX>(R(G.gamma), UL, UR, Nv, F);
             sgn = isL ? -1. : 1.;
         } else {
-            if (d == 0) riemann<FLUX, 0>(G.gamma, UL, UR, G.farea[0], F);
-            else if (d == 1) riemann<FLUX, 1>(G.gamma, UL, UR, G.farea[1], F);
-            else riemann<FLUX, 2>(G.gamma, UL, UR, G.farea[2], F);
+            if (d == 0) riemann<FLUX, 0>(R(G.gamma), UL, UR, R(G.farea[0]), F);
+            else if (d == 1) riemann<FLUX, 1>(R(G.gamma), UL, UR, R(G.farea[1]), F);
+            else riemann<FLUX, 2>(R(G.gamma), UL, UR, R(G.farea[2]), F);
         }
-        const double wq = LT.w[a1] * LT.w[a2];
-        double* o = fh + (((d * 2 + side) * M + a2) * M + a1) * V;
+        const R wq = LT.w[a1] * LT.w[a2];
+        R* o = fh + (((d * 2 + side) * M + a2) * M + a1) * V;
 #pragma unroll
         for (int s = 0; s < V; ++s) o[s] = sgn * (F[s] * wq);
     });
@@ -849,28 +887,28 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
     // 5. lift: transpose of the interpolation, along t2 then t1
     team_for<6 * M * V, TS>(lane, [&](int t) {
         const int s = t % V, u = t / V, a1 = u % M, ds = u / M;
-        double f[M];
+        R f[M];
 #pragma unroll
         for (int a2 = 0; a2 < M; ++a2) f[a2] = fh[((ds * M + a2) * M + a1) * V + s];
 #pragma unroll
         for (int t2 = 0; t2 < N; ++t2) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
-            for (int a2 = 0; a2 < M; ++a2) acc = fma(T.B[a2][t2], f[a2], acc);
+            for (int a2 = 0; a2 < M; ++a2) acc = fma(tB<R>(T, a2, t2), f[a2], acc);
             l1[((ds * N + t2) * M + a1) * V + s] = acc;
         }
     });
     sync();
     team_for<6 * N * V, TS>(lane, [&](int t) {
         const int s = t % V, u = t / V, t2 = u % N, ds = u / N;
-        double f[M];
+        R f[M];
 #pragma unroll
         for (int a1 = 0; a1 < M; ++a1) f[a1] = l1[((ds * N + t2) * M + a1) * V + s];
 #pragma unroll
         for (int t1 = 0; t1 < N; ++t1) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
-            for (int a1 = 0; a1 < M; ++a1) acc = fma(T.B[a1][t1], f[a1], acc);
+            for (int a1 = 0; a1 < M; ++a1) acc = fma(tB<R>(T, a1, t1), f[a1], acc);
             l2[((ds * N + t2) * N + t1) * V + s] = acc;
         }
     });
@@ -878,7 +916,7 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
     // 6. expand along the normal directions into the residual
     team_for<NNN * V, TS>(lane, [&](int t) {
         const int s = t % V, b = t / V, i = b % N, j = (b / N) % N, k = b / NN;
-        double acc = sR[t];
+        R acc = sR[t];
         // d = 0: r = i, (t1, t2) = (j, k)
         acc += LT.phim[i] * l2[((0 * N + k) * N + j) * V + s] + LT.phip[i] * l2[((1 * N + k) * N + j) * V + s];
         // d = 1: r = j, (t1, t2) = (i, k)
@@ -893,31 +931,31 @@ __device__ __forceinline__ void face_phase(const KTables& T, const LaneTab& LT, 
 // ---------------------------------------------------------------------------
 // Volume phase: sum-factorised interpolation, point fluxes, weak gradient.
 // ---------------------------------------------------------------------------
-template <int P, int TS, bool MAPPED = false>
-__device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT, const KGeom& G, int lane,
-                                             double* sR, double* bankA, double* bankB,
+template <int P, int TS, bool MAPPED = false, class R = double>
+__device__ __forceinline__ void volume_phase(const KTables& T, const LaneTabT<R>& LT, const KGeom& G, int lane,
+                                             R* sR, R* bankA, R* bankB,
                                              const double* XL = nullptr) {
     using Dm = Dims<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, MMM = Dm::MMM;
-    const double* sU = bankA;
-    double* X1 = bankB;
-    double* X2 = bankA;
-    double* Uq = bankB;
-    double* Qd = bankA;
-    double* Cc = bankB;
-    double* Bv = bankA;
+    const R* sU = bankA;
+    R* X1 = bankB;
+    R* X2 = bankA;
+    R* Uq = bankB;
+    R* Qd = bankA;
+    R* Cc = bankB;
+    R* Bv = bankA;
 
     // x: X1[k][j][a][s] = sum_i B[a][i] U[k][j][i][s]
     team_for<NN * V, TS>(lane, [&](int t) {
         const int s = t % V, kj = t / V;
-        double u[N];
+        R u[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) u[i] = sU[(kj * N + i) * V + s];
 #pragma unroll
         for (int a = 0; a < M; ++a) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
-            for (int i = 0; i < N; ++i) acc = fma(T.B[a][i], u[i], acc);
+            for (int i = 0; i < N; ++i) acc = fma(tB<R>(T, a, i), u[i], acc);
             X1[(kj * M + a) * V + s] = acc;
         }
     });
@@ -925,14 +963,14 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
     // y: X2[k][b][a][s] = sum_j B[b][j] X1[k][j][a][s]
     team_for<N * M * V, TS>(lane, [&](int t) {
         const int s = t % V, tt = t / V, a = tt % M, k = tt / M;
-        double u[N];
+        R u[N];
 #pragma unroll
         for (int j = 0; j < N; ++j) u[j] = X1[((k * N + j) * M + a) * V + s];
 #pragma unroll
         for (int b = 0; b < M; ++b) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
-            for (int j = 0; j < N; ++j) acc = fma(T.B[b][j], u[j], acc);
+            for (int j = 0; j < N; ++j) acc = fma(tB<R>(T, b, j), u[j], acc);
             X2[((k * M + b) * M + a) * V + s] = acc;
         }
     });
@@ -940,14 +978,14 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
     // z: Uq[c][b][a][s] = sum_k B[c][k] X2[k][b][a][s]
     team_for<MM * V, TS>(lane, [&](int t) {
         const int s = t % V, ba = t / V;
-        double u[N];
+        R u[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) u[k] = X2[(k * MM + ba) * V + s];
 #pragma unroll
         for (int c = 0; c < M; ++c) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
-            for (int k = 0; k < N; ++k) acc = fma(T.B[c][k], u[k], acc);
+            for (int k = 0; k < N; ++k) acc = fma(tB<R>(T, c, k), u[k], acc);
             Uq[(c * MM + ba) * V + s] = acc;
         }
     });
@@ -959,16 +997,16 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
     // flux_functors_impl.h:95-102 is W C_d.u + p C_d
     if constexpr (MAPPED) {
         team_for<MMM, TS>(lane, [&](int q) {
-            const double* u = Uq + q * V;
-            double U[V];
+            const R* u = Uq + q * V;
+            R U[V];
 #pragma unroll
             for (int s = 0; s < V; ++s) U[s] = u[s];
-            const double r1 = frcp(U[0]);
-            const double p = pressure_r(G.gamma, U, r1);
+            const R r1 = frcp(U[0]);
+            const R p = pressure_r(R(G.gamma), U, r1);
             const int a = q % M, b = (q / M) % M, c = q / MM;
-            double J[3][3];
+            R J[3][3];
             trilinear(XL, LT.xv[a], LT.xv[b], LT.xv[c], J);
-            double* o = Qd + q * 17;
+            R* o = Qd + q * 17;
             o[0] = U[0];
             o[1] = U[1];
             o[2] = U[2];
@@ -976,7 +1014,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
             o[4] = U[4] + p;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                double Cd[3];
+                R Cd[3];
                 cross3(J[(d + 1) % 3], J[(d + 2) % 3], Cd);
                 o[5 + d] = (Cd[0] * U[1] + Cd[1] * U[2] + Cd[2] * U[3]) * r1;
                 o[8 + 3 * d] = p * Cd[0];
@@ -986,13 +1024,13 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
         });
     } else
     team_for<MMM, TS>(lane, [&](int q) {
-        const double* u = Uq + q * V;
-        double U[V];
+        const R* u = Uq + q * V;
+        R U[V];
 #pragma unroll
         for (int s = 0; s < V; ++s) U[s] = u[s];
-        const double r1 = frcp(U[0]);
-        const double p = pressure_r(G.gamma, U, r1);
-        double* o = Qd + q * 9;
+        const R r1 = frcp(U[0]);
+        const R p = pressure_r(R(G.gamma), U, r1);
+        R* o = Qd + q * 9;
         o[0] = U[0];
         o[1] = U[1];
         o[2] = U[2];
@@ -1008,38 +1046,38 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
     // along z: B for x and y fluxes, D for the z flux
     team_for<MM * V, TS>(lane, [&](int t) {
         const int s = t % V, ba = t / V, a = ba % M, b = ba / M;
-        const double wab = LT.w[a] * LT.w[b];
+        const R wab = LT.w[a] * LT.w[b];
         // pressure enters the momentum component s of flux direction s - 1
-        const double e1 = s == 1 ? 1. : 0., e2 = s == 2 ? 1. : 0., e3 = s == 3 ? 1. : 0.;
-        double xx[N], yy[N], zz[N];
+        const R e1 = s == 1 ? 1. : 0., e2 = s == 2 ? 1. : 0., e3 = s == 3 ? 1. : 0.;
+        R xx[N], yy[N], zz[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) xx[k] = yy[k] = zz[k] = 0.;
 #pragma unroll
         for (int c = 0; c < M; ++c) {
-            const double wq = wab * T.w[c];
-            double gx, gy, gz;
+            const R wq = wab * tw<R>(T, c);
+            R gx, gy, gz;
             if constexpr (MAPPED) {
-                const double* q = Qd + (c * MM + ba) * 17;
-                const double Ws = q[s];
+                const R* q = Qd + (c * MM + ba) * 17;
+                const R Ws = q[s];
                 // p C_d . e_s for the momentum components
-                const double px = fma(e1, q[8], fma(e2, q[9], e3 * q[10]));
-                const double py = fma(e1, q[11], fma(e2, q[12], e3 * q[13]));
-                const double pz = fma(e1, q[14], fma(e2, q[15], e3 * q[16]));
+                const R px = fma(e1, q[8], fma(e2, q[9], e3 * q[10]));
+                const R py = fma(e1, q[11], fma(e2, q[12], e3 * q[13]));
+                const R pz = fma(e1, q[14], fma(e2, q[15], e3 * q[16]));
                 gx = fma(Ws, q[5], px) * wq;
                 gy = fma(Ws, q[6], py) * wq;
                 gz = fma(Ws, q[7], pz) * wq;
             } else {
-                const double* q = Qd + (c * MM + ba) * 9;
-                const double Ws = q[s], p = q[8];
-                gx = fma(Ws, q[5], e1 * p) * (G.gvol[0] * wq);
-                gy = fma(Ws, q[6], e2 * p) * (G.gvol[1] * wq);
-                gz = fma(Ws, q[7], e3 * p) * (G.gvol[2] * wq);
+                const R* q = Qd + (c * MM + ba) * 9;
+                const R Ws = q[s], p = q[8];
+                gx = fma(Ws, q[5], e1 * p) * (R(G.gvol[0]) * wq);
+                gy = fma(Ws, q[6], e2 * p) * (R(G.gvol[1]) * wq);
+                gz = fma(Ws, q[7], e3 * p) * (R(G.gvol[2]) * wq);
             }
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                xx[k] = fma(T.B[c][k], gx, xx[k]);
-                yy[k] = fma(T.B[c][k], gy, yy[k]);
-                zz[k] = fma(T.D[c][k], gz, zz[k]);
+                xx[k] = fma(tB<R>(T, c, k), gx, xx[k]);
+                yy[k] = fma(tB<R>(T, c, k), gy, yy[k]);
+                zz[k] = fma(tD<R>(T, c, k), gz, zz[k]);
             }
         }
 #pragma unroll
@@ -1053,7 +1091,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
     // b-step: along y (B for x and z fluxes, D for the y flux)
     team_for<N * M * V, TS>(lane, [&](int t) {
         const int s = t % V, tt = t / V, a = tt % M, k = tt / M;
-        double cx[M], cy[M], cz[M];
+        R cx[M], cy[M], cz[M];
 #pragma unroll
         for (int b = 0; b < M; ++b) {
             cx[b] = Cc[((0 * N + k) * MM + b * M + a) * V + s];
@@ -1062,12 +1100,12 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
         }
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            double y1 = 0., y2 = 0.;
+            R y1 = 0., y2 = 0.;
 #pragma unroll
             for (int b = 0; b < M; ++b) {
-                y1 = fma(T.B[b][j], cx[b], y1);
-                y2 = fma(T.D[b][j], cy[b], y2);
-                y2 = fma(T.B[b][j], cz[b], y2);
+                y1 = fma(tB<R>(T, b, j), cx[b], y1);
+                y2 = fma(tD<R>(T, b, j), cy[b], y2);
+                y2 = fma(tB<R>(T, b, j), cz[b], y2);
             }
             Bv[((0 * N + k) * N * M + j * M + a) * V + s] = y1;
             Bv[((1 * N + k) * N * M + j * M + a) * V + s] = y2;
@@ -1077,7 +1115,7 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
     // a-step: along x (D for the x flux, B for the others), into sR
     team_for<NN * V, TS>(lane, [&](int t) {
         const int s = t % V, kj = t / V;
-        double y1[M], y2[M];
+        R y1[M], y2[M];
 #pragma unroll
         for (int a = 0; a < M; ++a) {
             y1[a] = Bv[((0 * NN + kj) * M + a) * V + s];
@@ -1085,11 +1123,11 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTab& LT
         }
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            double acc = 0.;
+            R acc = 0.;
 #pragma unroll
             for (int a = 0; a < M; ++a) {
-                acc = fma(T.D[a][i], y1[a], acc);
-                acc = fma(T.B[a][i], y2[a], acc);
+                acc = fma(tD<R>(T, a, i), y1[a], acc);
+                acc = fma(tB<R>(T, a, i), y2[a], acc);
             }
             sR[(kj * N + i) * V + s] += acc;
         }
@@ -1160,8 +1198,8 @@ __device__ __forceinline__ double minv_diag_scale(const KTables& T, int b) {
 }
 
 // the same product from the LDS copy of the diagonal
-template <int P>
-__device__ __forceinline__ double lane_diag_scale(const LaneTab& LT, int b) {
+template <int P, class R>
+__device__ __forceinline__ double lane_diag_scale(const LaneTabT<R>& LT, int b) {
     constexpr int N = P + 1;
     const int i = b % N, j = (b / N) % N, k = b / (N * N);
     return LT.mdiag[i] * LT.mdiag[j] * LT.mdiag[k];
@@ -1295,17 +1333,19 @@ struct StageLoads {
     // neighbour slabs from the prefetched lines: the neighbour across the -
     // face contributes its + trace and vice versa; same fma chain as the
     // neighbour's own slab of that face
-    __device__ __forceinline__ void store_neighbour_slabs(const LaneTab& LT, double* slab, int lane) const {
+    // (FP32 compute: the line rounded to FP32 first, as the own block is)
+    template <class R>
+    __device__ __forceinline__ void store_neighbour_slabs(const LaneTabT<R>& LT, R* slab, int lane) const {
 #pragma unroll
         for (int r = 0; r < RN; ++r) {
             const int t = r * TS + lane;
             if (t < NT) {
                 const int s = t % V, u = t / V, t1 = u % N, u2 = u / N, t2 = u2 % N, ds = u2 / N;
                 const int d = ds >> 1, side = ds & 1;
-                const double* phi = side == 0 ? LT.phip : LT.phim;
-                double acc = 0.;
+                const R* phi = side == 0 ? LT.phip : LT.phim;
+                R acc = 0.;
 #pragma unroll
-                for (int q = 0; q < N; ++q) acc = fma(phi[q], nl[r][q], acc);
+                for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
                 slab[((d * 4 + 2 + side) * NN + t2 * N + t1) * V + s] = acc;
             }
         }
@@ -1355,23 +1395,29 @@ __device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int l
     sync();
 }
 
-template <int P, int FLUX, bool MAPPED>
+// R = float: the FP32 form -- the face and volume phases (traces, fluxes,
+// contractions) in single precision on FP32 copies of the state and the
+// tables; the residual is widened to FP64 before M^-1 and the RK update,
+// which read and write the FP64 registers.  Periodic box only.
+template <int P, int FLUX, bool MAPPED, class R = double>
 __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
+    static_assert(sizeof(R) == 8 || !MAPPED, "the FP32 form is for the periodic box");
     constexpr int TS = stage_team<P>();
     const KTables& T = *Tp;
     using Dm = Dims<P>;
     using L = StageLDS<P, MAPPED>;
     using PF = StageLoads<P, TS>;
     constexpr int NB = Dm::NNN * V;
-    __shared__ double lds[L::TOTAL];
-    __shared__ LaneTab LT;
+    static_assert(sizeof(R) == 8 || L::A * sizeof(R) >= NB * sizeof(double), "FP64 residual fits bank A");
+    __shared__ alignas(16) R lds[L::TOTAL];
+    __shared__ LaneTabT<R> LT;
     // general path: node coordinates of the element and of its six face
     // neighbours (slot d*2 + side), for the left elements' normals
     __shared__ double XL[MAPPED ? 168 : 1];
-    double* sR = lds;
-    double* bankA = lds + L::R;
-    double* bankB = bankA + L::A;
+    R* sR = lds;
+    R* bankA = lds + L::R;
+    R* bankB = bankA + L::A;
     const int lane = threadIdx.x;
     const unsigned e = block_element(G);
     const bool rk = A.mode != MODE_RESIDUAL;
@@ -1380,12 +1426,12 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     // global load of the element at once: its block, the neighbour lines
     // and the RK registers
     if (lane < PMAX + 2) {
-        LT.w[lane] = T.w[lane];
-        LT.xv[lane] = T.xv[lane];
+        LT.w[lane] = tw<R>(T, lane);
+        LT.xv[lane] = R(T.xv[lane]);
     }
     if (lane < PMAX + 1) {
-        LT.phim[lane] = T.phim[lane];
-        LT.phip[lane] = T.phip[lane];
+        LT.phim[lane] = tphim<R>(T, lane);
+        LT.phip[lane] = tphip<R>(T, lane);
         LT.mdiag[lane] = T.minv[lane][lane];
     }
     if constexpr (MAPPED) {
@@ -1406,15 +1452,28 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     for (int r = 0; r < PF::RU; ++r) {
         const int t = r * TS + lane;
         if (t < NB) {
-            bankA[t] = pf.own[r];
-            sR[t] = 0.;
+            bankA[t] = R(pf.own[r]);
+            sR[t] = R(0);
         }
     }
     if constexpr (MAPPED) pf.store_neighbour_slabs_gen(LT, bankB, lane);
     else pf.store_neighbour_slabs(LT, bankB, lane);
     sync();
-    face_phase<P, FLUX, TS, MAPPED>(T, LT, G, lane, bankA, sR, bankA, bankB, XL, e);
-    volume_phase<P, TS, MAPPED>(T, LT, G, lane, sR, bankA, bankB, XL);
+    face_phase<P, FLUX, TS, MAPPED, R>(T, LT, G, lane, bankA, sR, bankA, bankB, XL, e);
+    volume_phase<P, TS, MAPPED, R>(T, LT, G, lane, sR, bankA, bankB, XL);
+    // FP32 form: the residual widened to FP64 (bank A is free again)
+    double* sRd;
+    if constexpr (sizeof(R) == 8) {
+        sRd = sR;
+    } else {
+        sRd = reinterpret_cast<double*>(bankA);
+#pragma unroll
+        for (int r = 0; r < PF::RU; ++r) {
+            const int t = r * TS + lane;
+            if (t < NB) sRd[t] = (double)sR[t];
+        }
+        sync();
+    }
 
     // Epilogue with one store site per array; the value and the target are
     // picked by selects on the (uniform) mode.  Written as one branch per
@@ -1425,7 +1484,7 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     if constexpr (MAPPED) {
         if (rk) dense_minv<P, TS>(G.imm + (size_t)e * Dm::NNN * Dm::NNN, lane, sR, bankA);
     } else {
-        if (rk && !T.minv_diag) minv_lines<P, TS>(T, lane, sR);
+        if (rk && !T.minv_diag) minv_lines<P, TS>(T, lane, sRd);
     }
     const double scale = MAPPED ? A.dt : A.dt * G.jinv;
     double* const out = rk ? A.Uout : A.R;
@@ -1434,7 +1493,7 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     for (int r = 0; r < PF::RU; ++r) {
         const int t = r * TS + lane;
         if (t < NB) {
-            const double rv = sR[t];
+            const double rv = sRd[t];
             const double sc = (!MAPPED && T.minv_diag) ? scale * lane_diag_scale<P>(LT, t / V) : scale;
             const double dU = sc * rv;
             const double u0 = pf.u0[r], ac = pf.ac[r];
@@ -1815,6 +1874,12 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
             }
         } else {
             return hipErrorInvalidValue;
+        }
+    } else if (g.fp32) {
+        switch (flux) {
+        case 0: hipLaunchKernelGGL((stage_kernel<P, 0, false, float>), grid, block, 0, st, a, t, g); break;
+        case 1: hipLaunchKernelGGL((stage_kernel<P, 1, false, float>), grid, block, 0, st, a, t, g); break;
+        default: hipLaunchKernelGGL((stage_kernel<P, 2, false, float>), grid, block, 0, st, a, t, g); break;
         }
     } else {
         switch (flux) {

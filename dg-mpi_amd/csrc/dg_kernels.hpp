@@ -26,6 +26,12 @@ struct KTables {
     double BF[PMAX + 1][PMAX + 1];  // 2D faces: phi_i at the (p+1)-point segment rule
     double wf[PMAX + 1];            // 2D faces: weights of that rule
     int minv_diag;                  // 1: minv is diagonal (Legendre)
+    // single-precision copies (round to nearest) for the FP32 stage kernel
+    float B32[PMAX + 2][PMAX + 1];
+    float D32[PMAX + 2][PMAX + 1];
+    float phim32[PMAX + 1];
+    float phip32[PMAX + 1];
+    float w32[PMAX + 2];
 };
 
 // Structured periodic box and physics constants.  dim = 2: nx x ny box of
@@ -44,6 +50,7 @@ struct KGeom {
     int tile[3];                    // log2 brick edges of the stage-kernel traversal
     // general path (warped box, meshes from files): trilinear elements with
     // face records instead of box neighbours
+    int fp32;                       // 1: the stage kernel computes in FP32 (box only)
     int mapped;                     // 1: the fields below are used
     int exact_normals;              // 1: face normals at the flux points
     const double* X;                // [E][8][3] node coordinates, node 4k+2j+i

@@ -336,6 +336,7 @@ int dg_get_info(const dg_solver* s, dg_info* info) {
         info->num_basis = (c.order + 1) * (c.order + 1) * (c.dim == 3 ? c.order + 1 : 1);
         info->num_state_vars = s->ns;
         info->dim = c.dim;
+        info->precision = c.fp32 ? 32 : 64;
         for (int d = 0; d < 3; ++d) {
             info->nelem[d] = c.nelem[d];
             info->lower[d] = c.lower[d];

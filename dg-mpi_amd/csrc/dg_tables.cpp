@@ -170,6 +170,17 @@ KTables make_tables(const CaseConfig& c) {
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j)
                 if (i != j) t.minv[i][j] = 0.0;
+    for (int a = 0; a < PMAX + 2; ++a) {
+        for (int i = 0; i < PMAX + 1; ++i) {
+            t.B32[a][i] = (float)t.B[a][i];
+            t.D32[a][i] = (float)t.D[a][i];
+        }
+        t.w32[a] = (float)t.w[a];
+    }
+    for (int i = 0; i < PMAX + 1; ++i) {
+        t.phim32[i] = (float)t.phim[i];
+        t.phip32[i] = (float)t.phip[i];
+    }
     return t;
 }
 
@@ -206,6 +217,7 @@ KGeom make_geometry(const CaseConfig& c) {
     g.gamma = c.gamma;
     g.R = c.gas_constant;
     g.mapped = (c.warp > 0.0 || c.mesh_input) ? 1 : 0;
+    g.fp32 = c.fp32 ? 1 : 0;
     g.exact_normals = c.exact_normals ? 1 : 0;
     g.xcd_remap = (E % 8 == 0 && E >= 8 * 64) ? 1 : 0;
     // Stage-kernel traversal in bricks of up to 8^3 elements (edge: the

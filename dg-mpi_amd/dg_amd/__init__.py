@@ -51,7 +51,7 @@ class Info(C.Structure):
                 ("gamma", C.c_double), ("gas_constant", C.c_double),
                 ("timestep", C.c_double), ("end_time", C.c_double), ("time", C.c_double),
                 ("num_iterations", C.c_int), ("stepper", C.c_int), ("flux", C.c_int),
-                ("basis", C.c_int), ("ic", C.c_int), ("dim", C.c_int)]
+                ("basis", C.c_int), ("ic", C.c_int), ("dim", C.c_int), ("precision", C.c_int)]
 
 
 _lib = None
@@ -265,13 +265,15 @@ class Solver:
 def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(-5., -5., -5.),
               upper=(5., 5., 5.), stepper="RK4", timestep=None, cfl=None, end=1.0, gamma=1.4,
               gas_constant=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0),
-              warp=0.0, normals="reference", box=True):
+              warp=0.0, normals="reference", box=True, precision=None):
     """TOML text of a periodic-box case (keys of the reference input decks;
     warp > 0 gives the non-affine box; two entries in nelem, lower and upper
     give the 2D case, Physics.dim = 2).  box=False: the case of a mesh
     passed to Solver(..., mesh=...), whose [Mesh] holds only the normals
-    placement and the periodic cell (lower, upper) of the exact solutions."""
+    placement and the periodic cell (lower, upper) of the exact solutions.
+    precision="FP32": Numerics.precision of the FP32 form (3D box)."""
     st = f"timestep = {timestep!r}" if timestep is not None else f"cfl = {cfl!r}"
+    prec = f'\nprecision = "{precision}"' if precision else ""
     arr = lambda v: "[" + ", ".join(repr(float(x)) for x in v) + "]"
     # 2: quadrilaterals (basis *Quad, 4 state variables)
     dim = len(nelem) if box else len(lower)
@@ -286,7 +288,7 @@ upper = {arr(upper)}
 
 [Numerics]
 order = {order}
-basis = "{basis}"
+basis = "{basis}"{prec}
 
 [Stepper]
 type = "{stepper}"

@@ -66,6 +66,9 @@ typedef struct {
     int dim;                 /* 3: hexahedra, 5 state variables; 2:
                                 quadrilaterals, 4 state variables (rho,
                                 rho u, rho v, rho E), nelem[2] = 1           */
+    int precision;           /* 64, or 32: Numerics.precision = "FP32" (face
+                                and volume work of the stage in FP32; RK
+                                registers, M^-1 and reductions in FP64)      */
 } dg_info;
 
 /* Parse a TOML case (the keys of src/exec/main.cpp:38-83,

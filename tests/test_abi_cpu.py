@@ -227,3 +227,27 @@ def test_2d_mesh_input_is_checked_before_the_device():
     assert rc == 1 and "node index" in msg, msg
     with pytest.raises(ValueError, match="2D"):
         dg_amd.Solver(MESH_CASE2, mesh=(xyz, np.zeros((6, 8), dtype=np.int64), ifc))
+
+
+def test_precision_key():
+    """Numerics.precision (this build's key): FP64 (default) or FP32, the
+    FP32 form on the affine 3D box only."""
+    from dg_amd import hip
+    base = dg_amd.case_toml(order=2, nelem=(2, 2, 2), timestep=0.01, precision="FP32")
+    rc, msg = _create(base)
+    assert rc == (0 if hip.device_count() > 0 else 2), msg
+    rc, msg = _create(base.replace('precision = "FP32"', 'precision = "FP16"'))
+    assert rc == 1 and "FP16" in msg, msg
+    rc, msg = _create(base.replace("warp = 0.0", "warp = 0.1"))
+    assert rc == 4 and "precision = FP32 with Mesh.warp" in msg, msg
+
+
+def test_precision_fp32_not_on_2d_or_meshes():
+    text = BASE2.replace('basis = "LagrangeQuad"', 'basis = "LagrangeQuad"\nprecision = "FP32"')
+    rc, msg = _create(text)
+    assert rc == 4 and "Physics.dim = 2" in msg, msg
+    import meshgen
+    xyz, e2n, ifc = meshgen.box_mesh((2, 2, 2), (0, 0, 0), (1, 1, 1), m=3)
+    rc, msg = _create_mesh(MESH_CASE.replace('basis = "LagrangeHex"', 'basis = "LagrangeHex"\nprecision = "FP32"'),
+                           xyz, e2n, ifc)
+    assert rc == 4 and "on a mesh" in msg, msg
