@@ -114,6 +114,96 @@ def io_bench(s, path):
                       "header": h5.read_header(path)}))
 
 
+def batch_bench(args):
+    """Batched uneven cases: 256 independent periodic vortex problems (seed
+    args.seed) with n^3 elements, n uniform in 8..32, order p in {1..4},
+    vortex strength beta in [1, 5], free-stream Mach number in [0.2, 0.8]
+    along (1, 1, 0) (rho = p = 1), Legendre, Roe, RK4 at CFL 0.1, 0.1, 0.06,
+    0.04 for p = 1..4 (the dt = CFL h / max(|u| + c) rule is stable to these
+    values over the whole Mach range; at Mach 0.2-0.5 CFL 0.1 is not at
+    p = 3) -- advanced
+    by dg_batch_step (one launch per order and RK stage, steps replayed from
+    a captured HIP graph), against the same cases stepped one after the
+    other with dg_step; then the same batch in the FP32 form against the
+    FP64 one.  Prints one JSON line."""
+    import numpy as np
+    import dg_amd
+    from dg_amd import hip
+
+    rng = np.random.default_rng(args.seed)
+    n_cases = args.batch
+    cases = []
+    for _ in range(n_cases):
+        n = int(rng.integers(8, 33))
+        p = int(rng.integers(1, 5))
+        beta = float(rng.uniform(1.0, 5.0))
+        mach = float(rng.uniform(0.2, 0.8))
+        u = mach * np.sqrt(1.4) / np.sqrt(2.0)
+        cases.append((n, p, beta, mach, (1.0, u, u, 1.0, beta)))
+
+    def make(precision):
+        out = []
+        for n, p, beta, mach, data in cases:
+            s = dg_amd.Solver(dg_amd.case_toml(order=p, basis="LegendreHex", flux="Roe", nelem=(n, n, n),
+                                               cfl=(0.1, 0.1, 0.06, 0.04)[p - 1],
+                                               ic="ExtrudedIsentropicVortex", ic_data=data, precision=precision))
+            s.init_state_from_fcn()
+            out.append(s)
+        return out
+
+    dof = sum(n ** 3 * (p + 1) ** 3 for n, p, *_ in cases)
+    K, W = args.steps, args.warmup
+    def batch_time(solvers):
+        with dg_amd.Batch(solvers) as b:
+            b.step(W)
+            t0 = time.perf_counter()
+            b.step(K)
+            return (time.perf_counter() - t0) / K
+
+    s64 = make(None)
+    t64 = batch_time(s64)
+    e64 = np.array([s.get_error()[0] for s in s64])
+    s32 = make("FP32")
+    t32 = batch_time(s32)
+    e32 = np.array([s.get_error()[0] for s in s32])
+    d32 = np.array([float(np.max(np.abs(a.get_state() - b.get_state())) / np.max(np.abs(b.get_state())))
+                    for a, b in zip(s32, s64)])
+    for s in s32:
+        s.close()
+    # the same FP64 cases one after the other (dg_step per case), same steps
+    seq = make(None)
+    for s in seq:
+        s.take_time_step(0.0, W)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for s in seq:
+        s.take_time_step(0.0, K)
+    hip.synchronize()
+    t_seq = (time.perf_counter() - t0) / K
+    same = max(float(np.max(np.abs(a.get_state() - b.get_state())) / np.max(np.abs(b.get_state())))
+               for a, b in zip(s64, seq))
+    for s in s64 + seq:
+        s.close()
+    ratio = e32 / e64
+    out = {
+        "metric": "aggregate DOF-updates/s per RK stage, batched uneven periodic vortex cases",
+        "value": dof / (t64 / 4), "unit": "DOF-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": t64 * 1e3, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batch_{n_cases}_vortex_cases_n8-32_p1-4_legendre_roe_rk4_cfl0.1-0.04",
+                   "seed": args.seed, "cases": n_cases, "elements": int(sum(n ** 3 for n, *_ in cases)),
+                   "dof": int(dof), "orders": {str(q): sum(1 for c in cases if c[1] == q) for q in range(1, 5)}},
+        "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t64,
+        "max_rel_diff_vs_sequential": same,
+        "fp32_form": {"value": dof / (t32 / 4), "ms_per_step": t32 * 1e3, "speedup_vs_fp64": t64 / t32,
+                      "rho_l2_error_ratio_vs_fp64": {"median": float(np.median(ratio)), "max": float(np.max(ratio)),
+                                                     "min": float(np.min(ratio))},
+                      "state_max_rel_diff_vs_fp64": {"median": float(np.median(d32)), "max": float(np.max(d32))}},
+        "rho_l2_error_fp64": {"median": float(np.median(e64)), "max": float(np.max(e64)),
+                              "min": float(np.min(e64))},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +220,9 @@ def main():
     # the FP32 form only (for profiling its kernel); --no-fp32 skips it
     ap.add_argument("--fp32", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
+    # batched uneven cases (dg_batch_*): number of cases and their seed
+    ap.add_argument("--batch", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--seed", type=int, default=7, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -141,6 +234,9 @@ def main():
     from dg_amd import hip
 
     assert hip.device_count() > 0, "bench.py needs a GPU"
+    if args.batch:
+        batch_bench(args)
+        return
     ne, order, dim = args.ne, args.order, args.dim
     if dim == 2 and ne == NE:
         ne = 1024                         # 2D: 1024^2 elements, the DOF count of the 3D headline at p=3

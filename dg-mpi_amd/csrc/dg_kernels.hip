@@ -651,12 +651,9 @@ __device__ void ic_eval(const KIC& ic, const KGeom& g, const double* x, double t
 // Element indexing on the periodic box
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ unsigned block_element(const KGeom& g) {
-    const unsigned w = blockIdx.x;
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous range of the traversal so neighbour blocks hit its L2.
-    const unsigned p = g.xcd_remap ? (w & 7u) * (g.E >> 3) + (w >> 3) : w;
-    // traversal position -> element: bricks of 2^lx x 2^ly x 2^lz elements,
+// traversal position -> element of a box case
+__device__ __forceinline__ unsigned brick_element(const KGeom& g, unsigned p) {
+    // bricks of 2^lx x 2^ly x 2^lz elements,
     // x fastest inside a brick and across bricks (all edges 1: e = p)
     const unsigned lx = g.tile[0], ly = g.tile[1], lz = g.tile[2];
     const unsigned lxy = lx + ly;
@@ -667,6 +664,14 @@ __device__ __forceinline__ unsigned block_element(const KGeom& g) {
     const unsigned y = (ty << ly) | ((q >> lx) & ((1u << ly) - 1u));
     const unsigned z = (tz << lz) | (q >> lxy);
     return x + (unsigned)g.ne[0] * (y + (unsigned)g.ne[1] * z);
+}
+
+__device__ __forceinline__ unsigned block_element(const KGeom& g) {
+    const unsigned w = blockIdx.x;
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of the traversal so neighbour blocks hit its L2.
+    const unsigned p = g.xcd_remap ? (w & 7u) * (g.E >> 3) + (w >> 3) : w;
+    return brick_element(g, p);
 }
 
 __device__ __forceinline__ void elem_ijk(unsigned e, const KGeom& g, int* c) {
@@ -1399,12 +1404,10 @@ __device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int l
 // contractions) in single precision on FP32 copies of the state and the
 // tables; the residual is widened to FP64 before M^-1 and the RK update,
 // which read and write the FP64 registers.  Periodic box only.
-template <int P, int FLUX, bool MAPPED, class R = double>
-__global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
-                                                                KGeom G) {
+template <int P, int FLUX, bool MAPPED, class R>
+__device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T, const KGeom& G, unsigned e) {
     static_assert(sizeof(R) == 8 || !MAPPED, "the FP32 form is for the periodic box");
     constexpr int TS = stage_team<P>();
-    const KTables& T = *Tp;
     using Dm = Dims<P>;
     using L = StageLDS<P, MAPPED>;
     using PF = StageLoads<P, TS>;
@@ -1419,7 +1422,6 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
     R* bankA = lds + L::R;
     R* bankB = bankA + L::A;
     const int lane = threadIdx.x;
-    const unsigned e = block_element(G);
     const bool rk = A.mode != MODE_RESIDUAL;
 
     // lane tables first (their loads are the first to return), then every
@@ -1505,6 +1507,47 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
             if (wacc) A.acc[base + t] = mode == MODE_RK_FIRST ? dU : ac + 2.0 * dU;
         }
     }
+}
+
+template <int P, int FLUX, bool MAPPED, class R = double>
+__global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
+                                                                KGeom G) {
+    stage_body<P, FLUX, MAPPED, R>(A, *Tp, G, block_element(G));
+}
+
+// Batched stage: the elements of many independent box cases of one order
+// and flux in one launch.  The grid (a multiple of 8 blocks) is remapped
+// so that each XCD walks a contiguous eighth of the concatenated case
+// traversals; position q of the sequence is element brick_element(q -
+// first_block) of case blk_case[q].  Each case brings its own geometry,
+// tables and RK registers (BatchCase), so every element sees the stage
+// arithmetic of its case's own launch.  Positions past the last element
+// (the padding to a multiple of 8) return at once.
+template <int P, int FLUX, class R = double>
+__global__ __launch_bounds__(stage_team<P>()) void stage_batch_kernel(const BatchCase* __restrict__ cases,
+                                                                      const unsigned* __restrict__ blk_case,
+                                                                      unsigned nreal, int stage,
+                                                                      const KTables* __restrict__ Tp,
+                                                                      const char* base) {
+    const unsigned w = blockIdx.x;
+    const unsigned q = (w & 7u) * (gridDim.x >> 3) + (w >> 3);
+    if (q >= nreal) return;
+    const BatchCase* C = cases + blk_case[q];
+    // the RK4 stages of step_once: (Ust, Uout) = (U0, Ua), (Ua, Ub), (Ub, Ua), (Ua, U0)
+    const long long ou0 = C->off_U0, oua = C->off_Ua, oub = C->off_Ub;
+    const long long ost = stage == 0 ? ou0 : (stage == 2 ? oub : oua);
+    const long long oout = stage == 0 ? oua : (stage == 1 ? oub : (stage == 2 ? oua : ou0));
+    StageArgs A;
+    A.Ust = reinterpret_cast<const double*>(base + ost);
+    A.U0 = reinterpret_cast<const double*>(base + ou0);
+    A.Uout = reinterpret_cast<double*>(const_cast<char*>(base) + oout);
+    A.acc = reinterpret_cast<double*>(const_cast<char*>(base) + C->off_acc);
+    A.R = nullptr;
+    A.dt = C->dt;
+    A.alpha = stage == 2 ? 1.0 : 0.5;
+    A.mode = stage == 0 ? MODE_RK_FIRST : (stage == 3 ? MODE_RK_LAST : MODE_RK_MID);
+    const KGeom G = C->g;
+    stage_body<P, FLUX, false, R>(A, *Tp, G, brick_element(G, q - C->first_block));
 }
 
 // dU = dt M^-1 R
@@ -1891,6 +1934,26 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
     return hipGetLastError();
 }
 
+template <int P>
+hipError_t stage_batch_p(int flux, int fp32, const BatchCase* cases, const unsigned* blk_case, unsigned nblocks,
+                         int stage, const KTables* t, hipStream_t st) {
+    const dim3 grid((nblocks + 7u) & ~7u), block(stage_team<P>());
+    if (fp32) {
+        switch (flux) {
+        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, float>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, float>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, float>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        }
+    } else {
+        switch (flux) {
+        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        }
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 #define DG_DISPATCH_P(P, CALL)                         \
@@ -1909,6 +1972,13 @@ hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, c
         DG_DISPATCH_P(P, return stage2d_p<PP>(flux, a, t, g, st));
     }
     DG_DISPATCH_P(P, return stage_p<PP>(flux, a, t, g, st));
+    return hipSuccess;
+}
+
+hipError_t launch_stage_batch(int P, int flux, int fp32, const BatchCase* cases, const unsigned* blk_case,
+                              unsigned nblocks, int stage, const KTables* t, hipStream_t st) {
+    if (nblocks == 0) return hipSuccess;
+    DG_DISPATCH_P(P, return stage_batch_p<PP>(flux, fp32, cases, blk_case, nblocks, stage, t, st));
     return hipSuccess;
 }
 

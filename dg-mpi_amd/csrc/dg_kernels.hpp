@@ -85,11 +85,29 @@ struct StageArgs {
     int mode;
 };
 
+// One case of a batched stage launch (device memory): its RK registers as
+// byte offsets from the base pointer the launch receives (pointers derived
+// from a kernel argument are known to the compiler to be global memory,
+// unlike pointers read from memory), its time step, its geometry and the
+// first position of the launch that works on it (positions first_block ..
+// first_block + E - 1).
+struct BatchCase {
+    long long off_U0, off_Ua, off_Ub, off_acc;
+    double dt;
+    KGeom g;
+    unsigned first_block;
+};
+
 // Launchers; P in 1..PMAX, flux 0 LLF / 1 HLLC / 2 Roe.  Return the launch
 // status (hipGetLastError after the launch).
 // `t` points to the case's KTables in device memory.
 hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, const KGeom& g,
                         hipStream_t st);
+// Batched stage over the cases of one (order, basis, flux, precision)
+// group (tables t shared): nblocks = total elements of the group,
+// blk_case[q] = case of traversal position q, stage 0..3 of RK4.
+hipError_t launch_stage_batch(int P, int flux, int fp32, const BatchCase* cases, const unsigned* blk_case,
+                              unsigned nblocks, int stage, const KTables* t, hipStream_t st);
 hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,
                            const KGeom& g, hipStream_t st);
 hipError_t launch_init_state(int P, double time, double* U, const KTables* t, const KGeom& g,

@@ -59,34 +59,40 @@ void hip_check(hipError_t e, const char* what) {
     }
 }
 
+// run f, map its exception to a status and leave the message in err
 template <class F>
-int guarded(dg_solver* s, F&& f) {
+int guarded_to(std::string& err, F&& f) {
     try {
         f();
-        if (s) s->err.clear();
+        err.clear();
         return DG_OK;
     } catch (const dg::InputError& e) {
-        if (s) s->err = e.what(); else g_create_error = e.what();
+        err = e.what();
         return DG_ERR_INPUT;
     } catch (const dg::UnsupportedError& e) {
-        if (s) s->err = e.what(); else g_create_error = e.what();
+        err = e.what();
         return DG_ERR_UNSUPPORTED;
     } catch (const DeviceError& e) {
-        if (s) s->err = e.what(); else g_create_error = e.what();
+        err = e.what();
         return DG_ERR_DEVICE;
     } catch (const std::invalid_argument& e) {
-        if (s) s->err = e.what(); else g_create_error = e.what();
+        err = e.what();
         return DG_ERR_ARGUMENT;
     } catch (const std::domain_error& e) {
-        if (s) s->err = e.what(); else g_create_error = e.what();
+        err = e.what();
         return DG_ERR_NUMERICS;
     } catch (const std::bad_alloc&) {
-        if (s) s->err = "host out of memory"; else g_create_error = "host out of memory";
+        err = "host out of memory";
         return DG_ERR_DEVICE;
     } catch (const std::exception& e) {
-        if (s) s->err = e.what(); else g_create_error = e.what();
+        err = e.what();
         return DG_ERR_INPUT;
     }
+}
+
+template <class F>
+int guarded(dg_solver* s, F&& f) {
+    return guarded_to(s ? s->err : g_create_error, f);
 }
 
 void need(const void* p, const char* what) {
@@ -594,5 +600,196 @@ int dg_set_stream(dg_solver* s, void* stream) {
         s->stream = stream ? (hipStream_t)stream : s->own_stream;
     });
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of independent cases (dg_batch_*)
+// ---------------------------------------------------------------------------
+
+struct dg_batch {
+    // cases of one (order, basis, flux, precision): one launch per RK stage
+    struct Group {
+        int P = 0, basis = 0, flux = 0, fp32 = 0;
+        unsigned nblocks = 0;
+        std::vector<int> members;             // indices into cases
+        dg::BatchCase* dcases = nullptr;      // device descriptors
+        unsigned* blk_case = nullptr;         // device: case (within the group) of each block
+    };
+    std::vector<dg_solver*> cases;
+    std::vector<Group> groups;
+    std::vector<double> dts;                  // time steps the descriptors were built with
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;               // one RK4 step of every group
+    hipGraphExec_t exec = nullptr;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+// descriptors of a group's cases: the case's registers (as byte offsets
+// from the descriptor array, the base the launch receives), dt, geometry
+void upload_group(dg_batch* b, dg_batch::Group& g) {
+    using namespace dg;
+    std::vector<BatchCase> h(g.members.size());
+    const char* base = reinterpret_cast<const char*>(g.dcases);
+    auto off = [&](const double* p) { return (long long)(reinterpret_cast<const char*>(p) - base); };
+    unsigned first = 0;
+    for (size_t k = 0; k < g.members.size(); ++k) {
+        const dg_solver* s = b->cases[g.members[k]];
+        BatchCase& c = h[k];
+        std::memset(&c, 0, sizeof(c));
+        c.off_U0 = off(s->U0);
+        c.off_Ua = off(s->Ua);
+        c.off_Ub = off(s->Ub);
+        c.off_acc = off(s->acc);
+        c.dt = s->dt;
+        c.g = s->geo;
+        c.first_block = first;
+        first += s->geo.E;
+    }
+    hip_check(hipMemcpy(g.dcases, h.data(), h.size() * sizeof(BatchCase), hipMemcpyHostToDevice),
+              "hipMemcpy(batch cases)");
+}
+
+void free_batch(dg_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& g : b->groups) {
+        if (g.dcases) (void)hipFree(g.dcases);
+        if (g.blk_case) (void)hipFree(g.blk_case);
+    }
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_batch_create(int num_cases, dg_solver* const* cases, dg_batch** out) {
+    if (out) *out = nullptr;
+    dg_batch* b = nullptr;
+    const int rc = guarded_to(g_batch_error, [&] {
+        need(cases, "cases");
+        need(out, "out");
+        if (num_cases < 1) throw std::invalid_argument("dg_batch_create: num_cases < 1");
+        b = new dg_batch();
+        for (int i = 0; i < num_cases; ++i) {
+            dg_solver* s = cases[i];
+            need(s, "case handle");
+            for (dg_solver* o : b->cases)
+                if (o == s) throw std::invalid_argument("dg_batch_create: a case appears twice");
+            if (i > 0 && s->device != b->cases[0]->device)
+                throw std::invalid_argument("dg_batch_create: the cases live on different devices");
+            if (s->cfg.dim != 3 || s->geo.mapped)
+                throw dg::UnsupportedError("dg_batch_create: case " + std::to_string(i) +
+                                           " is not a 3D affine periodic box (the batched stage covers those)");
+            if (s->cfg.stepper != dg::Stepper::RK4)
+                throw dg::UnsupportedError("dg_batch_create: case " + std::to_string(i) + " does not use RK4");
+            b->cases.push_back(s);
+        }
+        b->device = b->cases[0]->device;
+        hip_check(hipSetDevice(b->device), "hipSetDevice");
+        // group by (order, basis, flux, precision), in order of first
+        // appearance; the cases of a group share their reference-element
+        // tables (they depend on the order and the basis only)
+        for (int i = 0; i < num_cases; ++i) {
+            const dg_solver* s = b->cases[i];
+            const int flux = (int)s->cfg.flux, fp32 = s->geo.fp32, basis = (int)s->cfg.basis;
+            dg_batch::Group* g = nullptr;
+            for (auto& q : b->groups)
+                if (q.P == s->P && q.basis == basis && q.flux == flux && q.fp32 == fp32) g = &q;
+            if (!g) {
+                b->groups.emplace_back();
+                g = &b->groups.back();
+                g->P = s->P;
+                g->basis = basis;
+                g->flux = flux;
+                g->fp32 = fp32;
+            }
+            if ((unsigned long long)g->nblocks + s->geo.E > 0x7ffffff0ULL)
+                throw dg::UnsupportedError("dg_batch_create: more than 2^31 - 1 elements of one order");
+            g->members.push_back(i);
+            g->nblocks += s->geo.E;
+        }
+        hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& g : b->groups) {
+            hip_check(hipMalloc(&g.dcases, g.members.size() * sizeof(dg::BatchCase)), "hipMalloc(batch cases)");
+            hip_check(hipMalloc(&g.blk_case, (size_t)g.nblocks * sizeof(unsigned)), "hipMalloc(batch blocks)");
+            std::vector<unsigned> bc(g.nblocks);
+            size_t k = 0;
+            for (size_t m = 0; m < g.members.size(); ++m)
+                for (unsigned e = 0; e < b->cases[g.members[m]]->geo.E; ++e) bc[k++] = (unsigned)m;
+            hip_check(hipMemcpy(g.blk_case, bc.data(), bc.size() * sizeof(unsigned), hipMemcpyHostToDevice),
+                      "hipMemcpy(batch blocks)");
+            upload_group(b, g);
+        }
+        b->dts.resize(b->cases.size());
+        for (size_t i = 0; i < b->cases.size(); ++i) b->dts[i] = b->cases[i]->dt;
+        // one RK4 step of every group, captured once and replayed per step
+        hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipError_t le = hipSuccess;
+        for (int st = 0; st < 4 && le == hipSuccess; ++st)
+            for (auto& g : b->groups) {
+                le = dg::launch_stage_batch(g.P, g.flux, g.fp32, g.dcases, g.blk_case, g.nblocks, st,
+                                            b->cases[g.members[0]]->dtab, b->stream);
+                if (le != hipSuccess) break;
+            }
+        const hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
+        hip_check(le, "batched stage launch (capture)");
+        hip_check(ce, "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    });
+    if (rc != DG_OK) {
+        free_batch(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return DG_OK;
+}
+
+int dg_batch_step(dg_batch* b, int nsteps) {
+    return guarded_to(b ? b->err : g_batch_error, [&] {
+        need(b, "batch");
+        if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
+        hip_check(hipSetDevice(b->device), "hipSetDevice");
+        bool changed = false;
+        for (size_t i = 0; i < b->cases.size(); ++i) {
+            const dg_solver* s = b->cases[i];
+            if (!(s->dt > 0.0))
+                throw std::invalid_argument("dg_batch_step: case " + std::to_string(i) +
+                                            " has no time step (Stepper.timestep, or dg_init_state with Stepper.cfl)");
+            if (s->dt != b->dts[i]) changed = true;
+            // earlier work on the case's own stream comes first
+            hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        }
+        if (changed) {
+            for (auto& g : b->groups) upload_group(b, g);
+            for (size_t i = 0; i < b->cases.size(); ++i) b->dts[i] = b->cases[i]->dt;
+        }
+        for (int n = 0; n < nsteps; ++n) hip_check(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        for (dg_solver* s : b->cases)
+            for (int n = 0; n < nsteps; ++n) {   // as step_once advances it
+                s->time += s->dt / 2.0;
+                s->time += s->dt / 2.0;
+            }
+    });
+}
+
+int dg_batch_destroy(dg_batch* b) {
+    if (!b) return DG_OK;
+    free_batch(b);
+    delete b;
+    return DG_OK;
+}
+
+const char* dg_batch_last_error(const dg_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
 
 }  // extern "C"

@@ -35,7 +35,8 @@ STEPPER_NAMES = ["FE", "RK4"]
 EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get_info", "dg_get_mesh", "dg_init_state",
            "dg_set_state", "dg_get_state", "dg_state_device_ptr", "dg_residual",
            "dg_apply_inv_mass", "dg_step", "dg_solve", "dg_l2_error", "dg_integrals",
-           "dg_max_wavespeed", "dg_set_time", "dg_set_stream"]
+           "dg_max_wavespeed", "dg_set_time", "dg_set_stream", "dg_batch_create", "dg_batch_step",
+           "dg_batch_destroy", "dg_batch_last_error"]
 
 
 class DGError(RuntimeError):
@@ -86,6 +87,9 @@ def lib():
             "dg_max_wavespeed": [vp, dp],
             "dg_set_time": [vp, C.c_double],
             "dg_set_stream": [vp, vp],
+            "dg_batch_create": [C.c_int, C.POINTER(vp), C.POINTER(vp)],
+            "dg_batch_step": [vp, C.c_int],
+            "dg_batch_destroy": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -93,6 +97,8 @@ def lib():
             f.restype = C.c_int
         L.dg_last_error.argtypes = [vp]
         L.dg_last_error.restype = C.c_char_p
+        L.dg_batch_last_error.argtypes = [vp]
+        L.dg_batch_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -260,6 +266,39 @@ class Solver:
         """Load a data.h5 file (Solver::read_in_coefficients)."""
         from . import h5
         h5.read_solution(self, path, set_time)
+
+
+class Batch:
+    """Independent box cases advanced together (dg_batch_*): one launch per
+    (order, flux, precision) group and RK stage, steps replayed from a
+    captured HIP graph.  Borrows the Solver handles; close it first."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = C.c_void_p()
+        rc = lib().dg_batch_create(len(self.solvers), arr, C.byref(h))
+        if rc != 0:
+            raise DGError(rc, lib().dg_batch_last_error(None).decode())
+        self._h = h
+
+    def step(self, nsteps=1):
+        rc = lib().dg_batch_step(self._h, nsteps)
+        if rc != 0:
+            raise DGError(rc, lib().dg_batch_last_error(self._h).decode())
+        for s in self.solvers:
+            s.info = s._info()
+
+    def close(self):
+        if self._h:
+            lib().dg_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(-5., -5., -5.),

@@ -180,6 +180,27 @@ int dg_set_time(dg_solver* s, double t);
  * kernels on their own stream. */
 int dg_set_stream(dg_solver* s, void* stream);
 
+/* Batches of independent cases (no reference counterpart: the reference
+ * runs one case per process).  A batch advances many periodic-box cases of
+ * uneven size, order, flux and precision together: per RK stage one launch
+ * per (order, flux, precision) group covers the elements of all its cases,
+ * and one RK4 step of every group is captured once as a HIP graph and
+ * replayed.  Each case keeps its own handle, registers, time step
+ * (Stepper.timestep or the CFL-derived one) and time; after
+ * dg_batch_step(b, n) every case holds what dg_step(case, 0, n) would have
+ * produced (the same stage arithmetic; the compiler may fuse multiply-adds
+ * differently in the batched kernel, which moves round-off only).  Cases: distinct handles on one device, 3D affine
+ * boxes (no mesh, no warp), RK4.  The batch borrows the handles: destroy it
+ * before them.  dg_batch_step waits for each case's stream before it starts
+ * and returns when the steps are done. */
+typedef struct dg_batch dg_batch;
+int dg_batch_create(int num_cases, dg_solver* const* cases, dg_batch** out);
+int dg_batch_step(dg_batch* b, int nsteps);
+int dg_batch_destroy(dg_batch* b);
+/* Last error of the batch (NULL: of the last failed dg_batch_create on this
+ * thread).  Never returns NULL. */
+const char* dg_batch_last_error(const dg_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

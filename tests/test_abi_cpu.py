@@ -251,3 +251,15 @@ def test_precision_fp32_not_on_2d_or_meshes():
     rc, msg = _create_mesh(MESH_CASE.replace('basis = "LagrangeHex"', 'basis = "LagrangeHex"\nprecision = "FP32"'),
                            xyz, e2n, ifc)
     assert rc == 4 and "on a mesh" in msg, msg
+
+
+def test_batch_create_checks_arguments():
+    h = C.c_void_p()
+    rc = dg_amd.lib().dg_batch_create(0, None, C.byref(h))
+    assert rc == 3 and "cases is NULL" in dg_amd.lib().dg_batch_last_error(None).decode()
+    arr = (C.c_void_p * 1)(None)
+    rc = dg_amd.lib().dg_batch_create(0, arr, C.byref(h))
+    assert rc == 3 and "num_cases < 1" in dg_amd.lib().dg_batch_last_error(None).decode()
+    rc = dg_amd.lib().dg_batch_create(1, arr, C.byref(h))
+    assert rc == 3 and "case handle is NULL" in dg_amd.lib().dg_batch_last_error(None).decode()
+    assert dg_amd.lib().dg_batch_destroy(None) == 0
