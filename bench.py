@@ -204,6 +204,57 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def closure_bench(args):
+    """Learned closure (coarse p = 1 on a 32^3 periodic box): training rows
+    from FP64 p = 4 runs of seeded vortex / acoustic-pulse mixtures, the
+    network trained by the library's own kernels, then held-out seeds run
+    to t = 0.5 with plain p = 1, p = 1 with the closure and plain p = 3,
+    each compared with the p = 4 run projected to p = 1.  One JSON line."""
+    import numpy as np
+    from dg_amd import closure, hip
+
+    n = args.closure
+    t0 = time.perf_counter()
+    train_seeds = list(range(1000, 1000 + args.closure_seeds))
+    X, Y, rows = closure.generate(train_seeds, n, fine_order=4, snapshots=12, every=6)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m, hist = closure.fit(X, Y, rows, epochs=args.closure_epochs, batch=4096, lr=2e-3, seed=0)
+    t_fit = time.perf_counter() - t0
+    X.free()
+    Y.free()
+    # a priori: held-out snapshots, network vs the zero correction
+    Xv, Yv, rv = closure.generate([1, 2, 3, 4], n, fine_order=4, snapshots=12, every=6)
+    mse_net, mse_zero = closure.a_priori(m, Xv, Yv, rv)
+    Xv.free()
+    Yv.free()
+    bounds, relax = (None, 2.0), (1.0, 0.5, 0.25)
+    ev = closure.evaluate(m, [1, 2, 3, 4], n, 0.5, fine_order=4, bounds=bounds, relax=relax, compare_orders=(2, 3))
+    mean = lambda k: float(np.mean([r[k] for r in ev]))
+    keys = [("closure" if b is None else f"closure_b{b:g}") + ("" if a == 1.0 else f"_r{a:g}")
+            for b in bounds for a in relax]
+    out = {
+        "metric": "coarse p=1 density L2 error vs the p=4 run projected to p=1, held-out seeds, t = 0.5",
+        "config": {"workload": f"closure_{n}^3_p1_mlp70-128-128-40_bf16", "train_seeds": len(train_seeds),
+                   "rows": int(rows), "epochs": args.closure_epochs},
+        "generate_s": t_gen, "fit_s": t_fit, "loss_history": hist,
+        "target_rms_element_norm": m.target_rms,
+        "a_priori_heldout_mse": {"network": mse_net, "zero_correction": mse_zero,
+                                 "fraction_removed": 1.0 - mse_net / mse_zero},
+        "p1_rho_error": mean("p1_rho_error"), "p2_rho_error": mean("p2_rho_error"),
+        "p3_rho_error": mean("p3_rho_error"),
+        "closure": {k: {"rho_error": mean(f"{k}_rho_error"),
+                        "error_ratio_vs_p1": mean(f"{k}_rho_error") / mean("p1_rho_error"),
+                        "fallback_elements": int(sum(r[f"{k}_fallback_elements"] for r in ev)),
+                        "wall_per_time_unit_s": mean(f"{k}_wall_per_time")} for k in keys},
+        "wall_per_time_unit_s": {"p1": mean("p1_wall_per_time"), "p2": mean("p2_wall_per_time"),
+                                 "p3": mean("p3_wall_per_time"), "p4": mean("fine_wall_per_time")},
+        "per_seed": ev,
+    }
+    m.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +274,10 @@ def main():
     # batched uneven cases (dg_batch_*): number of cases and their seed
     ap.add_argument("--batch", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--seed", type=int, default=7, help=argparse.SUPPRESS)
+    # learned closure: coarse mesh edge, training seeds, epochs
+    ap.add_argument("--closure", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--closure-seeds", type=int, default=24, help=argparse.SUPPRESS)
+    ap.add_argument("--closure-epochs", type=int, default=8, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -236,6 +291,9 @@ def main():
     assert hip.device_count() > 0, "bench.py needs a GPU"
     if args.batch:
         batch_bench(args)
+        return
+    if args.closure:
+        closure_bench(args)
         return
     ne, order, dim = args.ne, args.order, args.dim
     if dim == 2 and ne == NE:

@@ -1,0 +1,712 @@
+// Learned closure for coarse (p = 1 Legendre) periodic-box runs: per-element
+// features and training targets, a small MLP (70 -> 128 -> 128 -> 40, GELU,
+// BF16 weights with FP32 accumulation) with its own forward, backward and
+// Adam kernels, and RK4 stepping of the coarse solver with the corrected
+// tendency dU/dt = M^-1 R(U) + C(U).  No reference counterpart.
+#include "dg_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dg_solver_impl.hpp"
+
+namespace {
+
+constexpr int NF = 70, NH = 128, NO = 40;      // features, hidden width, outputs
+constexpr int NP = NF * NH + NH + NH * NH + NH + NH * NO + NO;
+constexpr int OFF_W1 = 0, OFF_B1 = NF * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH;
+constexpr int OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH * NO;
+constexpr int TB = 256;
+
+struct ClosureError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hcheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        std::ostringstream o;
+        o << what << ": " << hipGetErrorString(e);
+        throw ClosureError(o.str());
+    }
+}
+
+unsigned nblk(long n) { return (unsigned)((n + TB - 1) / TB); }
+
+// ---------------------------------------------------------------------------
+// bf16 storage (round to nearest even), FP32 arithmetic
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint16_t to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float from_bf16(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_d(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// H[m][n] = act(sum_k A'[m][k] W[k][n] + b[n]); A' = (A - mean) * scale for
+// the first layer (mean, scale non-null); Z keeps the pre-activation
+template <bool ACT>
+__global__ void dense_fwd(const float* __restrict__ A, const float* __restrict__ mean,
+                          const float* __restrict__ scale, const uint16_t* __restrict__ W,
+                          const float* __restrict__ b, float* __restrict__ Z, float* __restrict__ H, long M, int K,
+                          int N) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= M * N) return;
+    const long m = t / N;
+    const int n = (int)(t % N);
+    const float* a = A + m * K;
+    float acc = b[n];
+    for (int k = 0; k < K; ++k) {
+        const float x = mean ? (a[k] - mean[k]) * scale[k] : a[k];
+        acc = fmaf(x, from_bf16(W[(size_t)k * N + n]), acc);
+    }
+    if (Z) Z[t] = acc;
+    H[t] = ACT ? gelu(acc) : acc;
+}
+
+// G_in[m][k] = (sum_n G[m][n] W[k][n]) gelu'(Z[m][k])
+__global__ void dense_bwd_in(const float* __restrict__ G, const uint16_t* __restrict__ W,
+                             const float* __restrict__ Z, float* __restrict__ Gin, long M, int K, int N) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= M * K) return;
+    const long m = t / K;
+    const int k = (int)(t % K);
+    const float* g = G + m * N;
+    const uint16_t* w = W + (size_t)k * N;
+    float acc = 0.f;
+    for (int n = 0; n < N; ++n) acc = fmaf(g[n], from_bf16(w[n]), acc);
+    Gin[t] = acc * gelu_d(Z[t]);
+}
+
+// dW[k][n] = sum_m A'[m][k] G[m][n], db[n] = sum_m G[m][n] (A' normalised as
+// in dense_fwd when mean is non-null)
+__global__ void dense_grad(const float* __restrict__ A, const float* __restrict__ mean,
+                           const float* __restrict__ scale, const float* __restrict__ G, float* __restrict__ dW,
+                           float* __restrict__ db, long M, int K, int N) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= (long)(K + 1) * N) return;
+    const int k = (int)(t / N), n = (int)(t % N);
+    float acc = 0.f;
+    if (k < K) {
+        for (long m = 0; m < M; ++m) {
+            const float x = mean ? (A[m * K + k] - mean[k]) * scale[k] : A[m * K + k];
+            acc = fmaf(x, G[m * N + n], acc);
+        }
+        dW[(size_t)k * N + n] = acc;
+    } else {
+        for (long m = 0; m < M; ++m) acc += G[m * N + n];
+        db[n] = acc;
+    }
+}
+
+// output gradient of the mean squared error in normalised target units:
+// G = 2 (O - Y / yscale) / (M NO); per-block partial sums of the squared error
+__global__ void loss_grad(const float* __restrict__ O, const float* __restrict__ Y,
+                          const float* __restrict__ yscale, float* __restrict__ G, double* __restrict__ part,
+                          long M) {
+    __shared__ double red[TB];
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    double e2 = 0.;
+    if (t < M * NO) {
+        const int n = (int)(t % NO);
+        const float d = O[t] - Y[t] / yscale[n];
+        G[t] = 2.0f * d / (float)(M * NO);
+        e2 = (double)d * d;
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = TB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// rows idx[i] of X (NF) and Y (NO) into contiguous mini-batch buffers
+__global__ void gather_rows(const float* __restrict__ X, const float* __restrict__ Y, const int* __restrict__ idx,
+                            float* __restrict__ Xb, float* __restrict__ Yb, long B, long nrows) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= B * (NF + NO)) return;
+    const long i = t / (NF + NO);
+    const int c = (int)(t % (NF + NO));
+    long r = idx[i];
+    r = r < 0 ? 0 : (r >= nrows ? nrows - 1 : r);
+    if (c < NF) Xb[i * NF + c] = X[r * NF + c];
+    else Yb[i * NO + (c - NF)] = Y[r * NO + (c - NF)];
+}
+
+// Gaussian jitter of the mini-batch features, noise standard deviations in
+// normalised units (raw units: noise / scale); counter-based hash RNG
+// (splitmix64) and the Box-Muller transform
+__device__ inline unsigned long long mix64(unsigned long long z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__global__ void jitter_kernel(float* __restrict__ Xb, const float* __restrict__ scale, float noise, long n,
+                              unsigned long long key) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const unsigned long long h = mix64(key ^ mix64((unsigned long long)t));
+    const float u1 = ((float)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
+    const float u2 = ((float)((h >> 16) & 0xffffffu) + 0.5f) * (1.0f / 16777216.0f);
+    const float g = sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+    Xb[t] += noise * g / scale[t % NF];
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters; the BF16 working
+// copy follows
+__global__ void adam_step(float* __restrict__ p, uint16_t* __restrict__ pb, const float* __restrict__ g,
+                          float* __restrict__ m1, float* __restrict__ m2, float lr, float b1, float b2, float eps,
+                          float c1, float c2, int n) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    const float q = p[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+    p[i] = q;
+    pb[i] = to_bf16(q);
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ p, uint16_t* __restrict__ pb, int n) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i < n) pb[i] = to_bf16(p[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-state features and targets (p = 1 Legendre periodic box)
+// ---------------------------------------------------------------------------
+
+// face mean of element block u (8 x 5) on face (d, side), variable s:
+// (1/4) sum_ab w_a w_b U(face point (a, b)), the (p+2) = 3-point rule
+__device__ double face_mean(const dg::KTables& T, const double* u, int d, int side, int s) {
+    const double* phi = side ? T.phip : T.phim;
+    double acc = 0.;
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) {
+            double v = 0.;
+            for (int k = 0; k < 2; ++k)
+                for (int j = 0; j < 2; ++j)
+                    for (int i = 0; i < 2; ++i) {
+                        const int b = i + 2 * j + 4 * k;
+                        // normal index and the two tangential ones
+                        const int r = d == 0 ? i : (d == 1 ? j : k);
+                        const int t1 = d == 0 ? j : i, t2 = d == 2 ? j : k;
+                        v += u[b * 5 + s] * phi[r] * T.B[a][t1] * T.B[c][t2];
+                    }
+            acc += T.w[a] * T.w[c] * v;
+        }
+    return 0.25 * acc;
+}
+
+// X[e] = own 40 coefficients, then for the faces x-, x+, y-, y+, z-, z+ the
+// jump of the face-mean state (neighbour's matching face minus own), 5 each
+__global__ void features_kernel(const double* __restrict__ U, float* __restrict__ X,
+                                const dg::KTables* __restrict__ Tp, dg::KGeom G) {
+    const long e = (long)blockIdx.x * TB + threadIdx.x;
+    if (e >= (long)G.E) return;
+    const dg::KTables& T = *Tp;
+    const double* u = U + e * 40;
+    float* x = X + e * NF;
+    for (int c = 0; c < 40; ++c) x[c] = (float)u[c];
+    const int nx = G.ne[0], ny = G.ne[1], nz = G.ne[2];
+    const int ix = (int)(e % nx), iy = (int)((e / nx) % ny), iz = (int)(e / ((long)nx * ny));
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            int c[3] = {ix, iy, iz};
+            const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
+            c[d] = side ? (c[d] + 1) % nd : (c[d] + nd - 1) % nd;
+            const long en = c[0] + (long)nx * (c[1] + (long)ny * c[2]);
+            const double* un = U + en * 40;
+            for (int s = 0; s < 5; ++s)
+                x[40 + (d * 2 + side) * 5 + s] =
+                    (float)(face_mean(T, un, d, 1 - side, s) - face_mean(T, u, d, side, s));
+        }
+}
+
+// coarse state = Legendre modes (i, j, k) <= 1 of the fine state (the L2
+// projection onto p = 1 for an orthogonal basis on affine elements)
+__global__ void truncate_kernel(const double* __restrict__ Uf, double* __restrict__ Uc, int Pf, long E) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= E * 40) return;
+    const long e = t / 40;
+    const int r = (int)(t % 40), b = r / 5, s = r % 5;
+    const int i = b & 1, j = (b >> 1) & 1, k = b >> 2, n = Pf + 1;
+    Uc[t] = Uf[(e * n * n * n + i + n * j + n * n * k) * 5 + s];
+}
+
+// Y = trunc(Df) - Dc, in FP32
+__global__ void target_kernel(const double* __restrict__ Df, const double* __restrict__ Dc, float* __restrict__ Y,
+                              int Pf, long E) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= E * 40) return;
+    const long e = t / 40;
+    const int r = (int)(t % 40), b = r / 5, s = r % 5;
+    const int i = b & 1, j = (b >> 1) & 1, k = b >> 2, n = Pf + 1;
+    Y[t] = (float)(Df[(e * n * n * n + i + n * j + n * n * k) * 5 + s] - Dc[t]);
+}
+
+// C = O * yscale (the network's normalised outputs, de-normalised)
+__global__ void descale_kernel(const float* __restrict__ O, const float* __restrict__ ys, float* __restrict__ C,
+                               long n) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t < n) C[t] = O[t] * ys[t % NO];
+}
+
+// fallback: elements whose correction norm exceeds the bound get none
+__global__ void bound_kernel(float* __restrict__ C, double bound, long E, unsigned* __restrict__ count) {
+    const long e = (long)blockIdx.x * TB + threadIdx.x;
+    if (e >= E) return;
+    float* c = C + e * NO;
+    double n2 = 0.;
+    for (int i = 0; i < NO; ++i) n2 += (double)c[i] * c[i];
+    if (!(sqrt(n2) <= bound)) {
+        for (int i = 0; i < NO; ++i) c[i] = 0.f;
+        atomicAdd(count, 1u);
+    }
+}
+
+// RK register update with dU = dU_R + dt C (the modes of the stage kernel)
+__global__ void rk_update_kernel(int mode, double alpha, double dt, const double* __restrict__ dUR,
+                                 const float* __restrict__ C, const double* __restrict__ U0, double* __restrict__ acc,
+                                 double* __restrict__ Uout, long n) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= n) return;
+    const double dU = dUR[t] + dt * (double)C[t];
+    const double u0 = U0[t];
+    if (mode == dg::MODE_RK_FIRST) {
+        Uout[t] = u0 + alpha * dU;
+        acc[t] = dU;
+    } else if (mode == dg::MODE_RK_MID) {
+        Uout[t] = u0 + alpha * dU;
+        acc[t] = acc[t] + 2.0 * dU;
+    } else {
+        Uout[t] = u0 + (1.0 / 6.0) * (acc[t] + dU);
+    }
+}
+
+thread_local std::string g_mlp_error;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// MLP object
+// ---------------------------------------------------------------------------
+struct dg_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* p = nullptr;          // master parameters [NP]
+    uint16_t* pb = nullptr;      // BF16 working copy
+    float* g = nullptr;          // gradient
+    float* m1 = nullptr;         // Adam moments
+    float* m2 = nullptr;
+    float* xs = nullptr;         // feature mean [NF], scale [NF], target scale [NO]
+    long cap = 0;                // rows the activation buffers hold
+    float *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *O = nullptr, *G3 = nullptr,
+          *G2 = nullptr, *G1 = nullptr, *Xb = nullptr, *Yb = nullptr;
+    double* part = nullptr;
+    long step = 0;
+    std::string err;
+};
+
+namespace {
+
+void free_acts(dg_mlp* m) {
+    for (float* q : {m->Z1, m->H1, m->Z2, m->H2, m->O, m->G3, m->G2, m->G1, m->Xb, m->Yb})
+        if (q) (void)hipFree(q);
+    if (m->part) (void)hipFree(m->part);
+    m->Z1 = m->H1 = m->Z2 = m->H2 = m->O = m->G3 = m->G2 = m->G1 = m->Xb = m->Yb = nullptr;
+    m->part = nullptr;
+    m->cap = 0;
+}
+
+void ensure_acts(dg_mlp* m, long rows) {
+    if (rows <= m->cap) return;
+    free_acts(m);
+    auto al = [&](float** q, long n) { hcheck(hipMalloc(q, (size_t)n * sizeof(float)), "hipMalloc(mlp)"); };
+    al(&m->Z1, rows * NH);
+    al(&m->H1, rows * NH);
+    al(&m->Z2, rows * NH);
+    al(&m->H2, rows * NH);
+    al(&m->O, rows * NO);
+    al(&m->G3, rows * NO);
+    al(&m->G2, rows * NH);
+    al(&m->G1, rows * NH);
+    al(&m->Xb, rows * NF);
+    al(&m->Yb, rows * NO);
+    hcheck(hipMalloc(&m->part, (size_t)nblk(rows * NO) * sizeof(double)), "hipMalloc(mlp)");
+    m->cap = rows;
+}
+
+// forward pass of rows X (raw features) into m->O (normalised outputs)
+void forward(dg_mlp* m, const float* X, long rows) {
+    hipStream_t st = m->stream;
+    const float* mean = m->xs;
+    const float* scale = m->xs + NF;
+    hipLaunchKernelGGL(dense_fwd<true>, dim3(nblk(rows * NH)), dim3(TB), 0, st, X, mean, scale, m->pb + OFF_W1,
+                       m->p + OFF_B1, m->Z1, m->H1, rows, NF, NH);
+    hipLaunchKernelGGL(dense_fwd<true>, dim3(nblk(rows * NH)), dim3(TB), 0, st, m->H1, nullptr, nullptr,
+                       m->pb + OFF_W2, m->p + OFF_B2, m->Z2, m->H2, rows, NH, NH);
+    hipLaunchKernelGGL(dense_fwd<false>, dim3(nblk(rows * NO)), dim3(TB), 0, st, m->H2, nullptr, nullptr,
+                       m->pb + OFF_W3, m->p + OFF_B3, nullptr, m->O, rows, NH, NO);
+    hcheck(hipGetLastError(), "mlp forward");
+}
+
+// loss and gradient (into m->g) of rows X, Y; returns the mean squared error
+double backward(dg_mlp* m, const float* X, const float* Y, long rows) {
+    hipStream_t st = m->stream;
+    const float* mean = m->xs;
+    const float* scale = m->xs + NF;
+    const float* ys = m->xs + 2 * NF;
+    forward(m, X, rows);
+    const unsigned nb = nblk(rows * NO);
+    hipLaunchKernelGGL(loss_grad, dim3(nb), dim3(TB), 0, st, m->O, Y, ys, m->G3, m->part, rows);
+    hipLaunchKernelGGL(dense_grad, dim3(nblk((long)(NH + 1) * NO)), dim3(TB), 0, st, m->H2, nullptr, nullptr, m->G3,
+                       m->g + OFF_W3, m->g + OFF_B3, rows, NH, NO);
+    hipLaunchKernelGGL(dense_bwd_in, dim3(nblk(rows * NH)), dim3(TB), 0, st, m->G3, m->pb + OFF_W3, m->Z2, m->G2,
+                       rows, NH, NO);
+    hipLaunchKernelGGL(dense_grad, dim3(nblk((long)(NH + 1) * NH)), dim3(TB), 0, st, m->H1, nullptr, nullptr, m->G2,
+                       m->g + OFF_W2, m->g + OFF_B2, rows, NH, NH);
+    hipLaunchKernelGGL(dense_bwd_in, dim3(nblk(rows * NH)), dim3(TB), 0, st, m->G2, m->pb + OFF_W2, m->Z1, m->G1,
+                       rows, NH, NH);
+    hipLaunchKernelGGL(dense_grad, dim3(nblk((long)(NF + 1) * NH)), dim3(TB), 0, st, X, mean, scale, m->G1,
+                       m->g + OFF_W1, m->g + OFF_B1, rows, NF, NH);
+    hcheck(hipGetLastError(), "mlp backward");
+    std::vector<double> h(nb);
+    hcheck(hipMemcpyAsync(h.data(), m->part, nb * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
+    hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    double s = 0.;
+    for (double v : h) s += v;
+    return s / (double)(rows * NO);
+}
+
+template <class F>
+int mlp_guard(std::string& err, F&& f) {
+    try {
+        f();
+        err.clear();
+        return DG_OK;
+    } catch (const std::invalid_argument& e) {
+        err = e.what();
+        return DG_ERR_ARGUMENT;
+    } catch (const dg::UnsupportedError& e) {
+        err = e.what();
+        return DG_ERR_UNSUPPORTED;
+    } catch (const std::domain_error& e) {
+        err = e.what();
+        return DG_ERR_NUMERICS;
+    } catch (const std::exception& e) {
+        err = e.what();
+        return DG_ERR_DEVICE;
+    }
+}
+
+void need_ptr(const void* p, const char* what) {
+    if (!p) throw std::invalid_argument(std::string(what) + " is NULL");
+}
+
+// the coarse / fine solvers this module works with: 3D Legendre periodic boxes
+void check_box(const dg_solver* s, const char* what, int order) {
+    need_ptr(s, what);
+    if (s->cfg.dim != 3 || s->geo.mapped || s->cfg.basis != dg::Basis::LegendreHex || s->geo.fp32)
+        throw dg::UnsupportedError(std::string(what) + ": a 3D affine FP64 LegendreHex periodic box is required");
+    if (order > 0 && s->P != order)
+        throw dg::UnsupportedError(std::string(what) + ": order " + std::to_string(order) + " is required");
+}
+
+// dD = M^-1 R(U) of solver s (dt = 1), with its own stage registers as scratch
+void tendency(dg_solver* s, const double* U, double* D) {
+    dg::StageArgs a{};
+    a.Ust = U;
+    a.R = s->Ua;
+    a.mode = dg::MODE_RESIDUAL;
+    hcheck(dg::launch_stage(s->P, (int)s->cfg.flux, a, s->dtab, s->geo, s->stream), "stage kernel");
+    hcheck(dg::launch_inv_mass(s->P, 1.0, s->Ua, D, s->dtab, s->geo, s->stream), "inverse mass kernel");
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_mlp_create(unsigned long long seed, dg_mlp** out) {
+    if (out) *out = nullptr;
+    dg_mlp* m = nullptr;
+    const int rc = mlp_guard(g_mlp_error, [&] {
+        need_ptr(out, "out");
+        m = new dg_mlp();
+        hcheck(hipGetDevice(&m->device), "hipGetDevice");
+        hcheck(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hcheck(hipMalloc(&m->p, NP * sizeof(float)), "hipMalloc(mlp)");
+        hcheck(hipMalloc(&m->pb, NP * sizeof(uint16_t)), "hipMalloc(mlp)");
+        hcheck(hipMalloc(&m->g, NP * sizeof(float)), "hipMalloc(mlp)");
+        hcheck(hipMalloc(&m->m1, NP * sizeof(float)), "hipMalloc(mlp)");
+        hcheck(hipMalloc(&m->m2, NP * sizeof(float)), "hipMalloc(mlp)");
+        hcheck(hipMalloc(&m->xs, (2 * NF + NO) * sizeof(float)), "hipMalloc(mlp)");
+        // He-normal hidden layers, a small output layer (the correction
+        // starts near zero), zero biases; identity scaling
+        std::mt19937_64 rng(seed);
+        std::normal_distribution<float> nd(0.f, 1.f);
+        std::vector<float> h(NP, 0.f);
+        for (int i = 0; i < NF * NH; ++i) h[OFF_W1 + i] = nd(rng) * std::sqrt(2.f / NF);
+        for (int i = 0; i < NH * NH; ++i) h[OFF_W2 + i] = nd(rng) * std::sqrt(2.f / NH);
+        for (int i = 0; i < NH * NO; ++i) h[OFF_W3 + i] = nd(rng) * 0.1f * std::sqrt(1.f / NH);
+        hcheck(hipMemcpy(m->p, h.data(), NP * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        std::vector<float> z(NP, 0.f);
+        hcheck(hipMemcpy(m->m1, z.data(), NP * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        hcheck(hipMemcpy(m->m2, z.data(), NP * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        std::vector<float> xs(2 * NF + NO, 1.f);
+        for (int i = 0; i < NF; ++i) xs[i] = 0.f;
+        hcheck(hipMemcpy(m->xs, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, NP);
+        hcheck(hipGetLastError(), "to_bf16");
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+    });
+    if (rc != DG_OK) {
+        if (m) dg_mlp_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return DG_OK;
+}
+
+int dg_mlp_destroy(dg_mlp* m) {
+    if (!m) return DG_OK;
+    (void)hipSetDevice(m->device);
+    free_acts(m);
+    for (float* q : {m->p, m->g, m->m1, m->m2, m->xs})
+        if (q) (void)hipFree(q);
+    if (m->pb) (void)hipFree(m->pb);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return DG_OK;
+}
+
+const char* dg_mlp_last_error(const dg_mlp* m) { return m ? m->err.c_str() : g_mlp_error.c_str(); }
+
+int dg_mlp_num_params(long* n) {
+    if (!n) return DG_ERR_ARGUMENT;
+    *n = NP;
+    return DG_OK;
+}
+
+int dg_mlp_get_params(const dg_mlp* m, float* host) {
+    return mlp_guard(const_cast<dg_mlp*>(m)->err, [&] {
+        need_ptr(host, "host");
+        hcheck(hipMemcpy(host, m->p, NP * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+int dg_mlp_set_params(dg_mlp* m, const float* host) {
+    return mlp_guard(m ? m->err : g_mlp_error, [&] {
+        need_ptr(m, "mlp");
+        need_ptr(host, "host");
+        hcheck(hipMemcpy(m->p, host, NP * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, NP);
+        hcheck(hipGetLastError(), "to_bf16");
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+    });
+}
+
+int dg_mlp_set_scaling(dg_mlp* m, const float* x_mean, const float* x_scale, const float* y_scale) {
+    return mlp_guard(m ? m->err : g_mlp_error, [&] {
+        need_ptr(m, "mlp");
+        need_ptr(x_mean, "x_mean");
+        need_ptr(x_scale, "x_scale");
+        need_ptr(y_scale, "y_scale");
+        std::vector<float> xs(2 * NF + NO);
+        for (int i = 0; i < NF; ++i) {
+            xs[i] = x_mean[i];
+            xs[NF + i] = x_scale[i];
+        }
+        for (int i = 0; i < NO; ++i) {
+            if (!(y_scale[i] > 0.f)) throw std::invalid_argument("y_scale entries must be > 0");
+            xs[2 * NF + i] = y_scale[i];
+        }
+        hcheck(hipMemcpy(m->xs, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+    });
+}
+
+int dg_mlp_forward(dg_mlp* m, const float* X_dev, float* Y_dev, long rows) {
+    return mlp_guard(m ? m->err : g_mlp_error, [&] {
+        need_ptr(m, "mlp");
+        need_ptr(X_dev, "X");
+        need_ptr(Y_dev, "Y");
+        if (rows < 1) throw std::invalid_argument("rows < 1");
+        hcheck(hipSetDevice(m->device), "hipSetDevice");
+        ensure_acts(m, rows);
+        forward(m, X_dev, rows);
+        hipLaunchKernelGGL(descale_kernel, dim3(nblk(rows * NO)), dim3(TB), 0, m->stream, m->O, m->xs + 2 * NF, Y_dev,
+                           rows * NO);
+        hcheck(hipGetLastError(), "descale kernel");
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+    });
+}
+
+int dg_mlp_gradient(dg_mlp* m, const float* X_dev, const float* Y_dev, long rows, float* grad_host, double* loss) {
+    return mlp_guard(m ? m->err : g_mlp_error, [&] {
+        need_ptr(m, "mlp");
+        need_ptr(X_dev, "X");
+        need_ptr(Y_dev, "Y");
+        if (rows < 1) throw std::invalid_argument("rows < 1");
+        hcheck(hipSetDevice(m->device), "hipSetDevice");
+        ensure_acts(m, rows);
+        const double l = backward(m, X_dev, Y_dev, rows);
+        if (loss) *loss = l;
+        if (grad_host) hcheck(hipMemcpy(grad_host, m->g, NP * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+int dg_mlp_train(dg_mlp* m, const float* X_dev, const float* Y_dev, long nrows, const int* idx_dev, long nidx,
+                 int batch, float lr, float noise, double* mean_loss) {
+    return mlp_guard(m ? m->err : g_mlp_error, [&] {
+        need_ptr(m, "mlp");
+        need_ptr(X_dev, "X");
+        need_ptr(Y_dev, "Y");
+        need_ptr(idx_dev, "idx");
+        if (nrows < 1 || nidx < 1 || batch < 1) throw std::invalid_argument("empty training set or batch");
+        hcheck(hipSetDevice(m->device), "hipSetDevice");
+        ensure_acts(m, batch);
+        double sum = 0.;
+        long nb = 0;
+        for (long i0 = 0; i0 + batch <= nidx; i0 += batch) {
+            hipLaunchKernelGGL(gather_rows, dim3(nblk((long)batch * (NF + NO))), dim3(TB), 0, m->stream, X_dev,
+                               Y_dev, idx_dev + i0, m->Xb, m->Yb, (long)batch, nrows);
+            if (noise > 0.f)
+                hipLaunchKernelGGL(jitter_kernel, dim3(nblk((long)batch * NF)), dim3(TB), 0, m->stream, m->Xb,
+                                   m->xs + NF, noise, (long)batch * NF, (unsigned long long)m->step * 0x100000001ull);
+            sum += backward(m, m->Xb, m->Yb, batch);
+            ++nb;
+            ++m->step;
+            const float b1 = 0.9f, b2 = 0.999f;
+            const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+            hipLaunchKernelGGL(adam_step, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, m->g, m->m1, m->m2, lr,
+                               b1, b2, 1e-8f, c1, c2, NP);
+            hcheck(hipGetLastError(), "adam");
+        }
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+        if (mean_loss) *mean_loss = nb ? sum / nb : 0.;
+    });
+}
+
+int dg_closure_features(dg_solver* s, const double* U_dev, float* X_dev) {
+    return mlp_guard(g_mlp_error, [&] {
+        check_box(s, "solver", 1);
+        need_ptr(U_dev, "U");
+        need_ptr(X_dev, "X");
+        hcheck(hipSetDevice(s->device), "hipSetDevice");
+        hipLaunchKernelGGL(features_kernel, dim3(nblk(s->geo.E)), dim3(TB), 0, s->stream, U_dev, X_dev, s->dtab,
+                           s->geo);
+        hcheck(hipGetLastError(), "features kernel");
+        hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    });
+}
+
+int dg_closure_target(dg_solver* fine, const double* Uf_dev, dg_solver* coarse, double* Uc_dev, float* Y_dev) {
+    return mlp_guard(g_mlp_error, [&] {
+        check_box(fine, "fine solver", 0);
+        check_box(coarse, "coarse solver", 1);
+        need_ptr(Uf_dev, "Uf");
+        need_ptr(Uc_dev, "Uc");
+        need_ptr(Y_dev, "Y");
+        if (fine->P < 2) throw std::invalid_argument("the fine solver's order must exceed 1");
+        for (int d = 0; d < 3; ++d)
+            if (fine->geo.ne[d] != coarse->geo.ne[d] || fine->geo.h[d] != coarse->geo.h[d] ||
+                fine->geo.lo[d] != coarse->geo.lo[d])
+                throw std::invalid_argument("fine and coarse solvers must share the mesh");
+        if (fine->device != coarse->device) throw std::invalid_argument("the solvers live on different devices");
+        hcheck(hipSetDevice(fine->device), "hipSetDevice");
+        hcheck(hipStreamSynchronize(coarse->stream), "hipStreamSynchronize");
+        const long E = fine->geo.E;
+        // fine tendency into the fine handle's Ub, then the coarse state and its tendency
+        tendency(fine, Uf_dev, fine->Ub);
+        hipLaunchKernelGGL(truncate_kernel, dim3(nblk(E * 40)), dim3(TB), 0, fine->stream, Uf_dev, Uc_dev, fine->P, E);
+        hcheck(hipGetLastError(), "truncate kernel");
+        hcheck(hipStreamSynchronize(fine->stream), "hipStreamSynchronize");
+        tendency(coarse, Uc_dev, coarse->Ub);
+        hcheck(hipStreamSynchronize(coarse->stream), "hipStreamSynchronize");
+        hipLaunchKernelGGL(target_kernel, dim3(nblk(E * 40)), dim3(TB), 0, fine->stream, fine->Ub, coarse->Ub, Y_dev,
+                           fine->P, E);
+        hcheck(hipGetLastError(), "target kernel");
+        hcheck(hipStreamSynchronize(fine->stream), "hipStreamSynchronize");
+    });
+}
+
+int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int nsteps, long* n_fallback) {
+    return mlp_guard(g_mlp_error, [&] {
+        check_box(s, "solver", 1);
+        need_ptr(m, "mlp");
+        if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
+        const double dt = dt_step > 0.0 ? dt_step : s->dt;
+        if (!(dt > 0.0)) throw std::invalid_argument("no time step");
+        if (s->cfg.stepper != dg::Stepper::RK4) throw dg::UnsupportedError("the closure steps RK4 only");
+        hcheck(hipSetDevice(s->device), "hipSetDevice");
+        const long E = s->geo.E, n = E * 40;
+        float *X = nullptr, *C = nullptr;
+        double* dU = nullptr;
+        unsigned* cnt = nullptr;
+        hcheck(hipMalloc(&X, (size_t)E * NF * sizeof(float)), "hipMalloc");
+        hcheck(hipMalloc(&C, (size_t)n * sizeof(float)), "hipMalloc");
+        hcheck(hipMalloc(&dU, (size_t)n * sizeof(double)), "hipMalloc");
+        hcheck(hipMalloc(&cnt, sizeof(unsigned)), "hipMalloc");
+        hcheck(hipMemset(cnt, 0, sizeof(unsigned)), "hipMemset");
+        ensure_acts(m, E);
+        auto stage = [&](const double* Ust, double* Uout, int mode, double alpha) {
+            hipLaunchKernelGGL(features_kernel, dim3(nblk(E)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
+            hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+            forward(m, X, E);
+            hipLaunchKernelGGL(descale_kernel, dim3(nblk(n)), dim3(TB), 0, m->stream, m->O, m->xs + 2 * NF, C, n);
+            hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+            hipLaunchKernelGGL(bound_kernel, dim3(nblk(E)), dim3(TB), 0, s->stream, C, bound, E, cnt);
+            dg::StageArgs a{};
+            a.Ust = Ust;
+            a.R = s->Ub;
+            a.mode = dg::MODE_RESIDUAL;
+            hcheck(dg::launch_stage(s->P, (int)s->cfg.flux, a, s->dtab, s->geo, s->stream), "stage kernel");
+            hcheck(dg::launch_inv_mass(s->P, dt, s->Ub, dU, s->dtab, s->geo, s->stream), "inverse mass kernel");
+            hipLaunchKernelGGL(rk_update_kernel, dim3(nblk(n)), dim3(TB), 0, s->stream, mode, alpha, dt, dU, C, s->U0,
+                               s->acc, Uout, n);
+            hcheck(hipGetLastError(), "closure stage");
+        };
+        for (int k = 0; k < nsteps; ++k) {
+            // stages read U0 / Ua and write Ua / U0; the residual goes to Ub
+            stage(s->U0, s->Ua, dg::MODE_RK_FIRST, 0.5);
+            s->time += dt / 2.0;
+            stage(s->Ua, s->Ua, dg::MODE_RK_MID, 0.5);
+            stage(s->Ua, s->Ua, dg::MODE_RK_MID, 1.0);
+            s->time += dt / 2.0;
+            stage(s->Ua, s->U0, dg::MODE_RK_LAST, 0.0);
+        }
+        hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        unsigned c = 0;
+        hcheck(hipMemcpy(&c, cnt, sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (n_fallback) *n_fallback = (long)c;
+        (void)hipFree(X);
+        (void)hipFree(C);
+        (void)hipFree(dU);
+        (void)hipFree(cnt);
+    });
+}
+
+}  // extern "C"

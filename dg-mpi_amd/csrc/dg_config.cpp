@@ -285,9 +285,10 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     // enums.h:93-107 plus three new names
     {
         int idx = pick(icn, {"Uniform", "Gaussian", "Sphere", "IsentropicVortex", "ExtrudedIsentropicVortex",
-                             "GaussianBump", "TaylorGreen"});
+                             "GaussianBump", "TaylorGreen", "VortexPulseMix"});
         static const IC map[] = {IC::Uniform, IC::Gaussian, IC::Sphere, IC::IsentropicVortex,
-                                 IC::ExtrudedIsentropicVortex, IC::GaussianBump, IC::TaylorGreen};
+                                 IC::ExtrudedIsentropicVortex, IC::GaussianBump, IC::TaylorGreen,
+                                 IC::VortexPulseMix};
         c.ic = map[idx];
         // Physics<2>::call_IC and Physics<3>::call_IC (physics/base/base.cpp:77-103)
         if (dim == 2 && !(c.ic == IC::Uniform || c.ic == IC::Gaussian || c.ic == IC::IsentropicVortex))
@@ -296,7 +297,9 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
             throw UnsupportedError("InitialCondition.name = Gaussian is a 2D initial condition");
     }
     c.ic_data = get_num_array(d, "InitialCondition", "data", false);
-    if (c.ic_data.size() > 10) throw InputError("InitialCondition.data holds at most 10 values");
+    if (c.ic_data.size() > 24) throw InputError("InitialCondition.data holds at most 24 values");
+    if (c.ic != IC::VortexPulseMix && c.ic_data.size() > 10)
+        throw InputError("InitialCondition.data holds at most 10 values");
     size_t need = 0;
     switch (c.ic) {
     case IC::Uniform: need = dim == 2 ? 4 : 5; break;
@@ -305,11 +308,13 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     case IC::ExtrudedIsentropicVortex: need = 5; break;
     case IC::GaussianBump: need = 9; break;
     case IC::TaylorGreen: need = 1; break;
+    case IC::VortexPulseMix: need = 19; break;
     default: need = 0;
     }
     if (c.ic_data.size() < need)
         throw InputError("InitialCondition.data needs " + std::to_string(need) + " values for " + icn);
-    if (((dim == 2 && c.ic == IC::IsentropicVortex) || c.ic == IC::ExtrudedIsentropicVortex) && c.gas_constant != 1.0)
+    if (((dim == 2 && c.ic == IC::IsentropicVortex) || c.ic == IC::ExtrudedIsentropicVortex ||
+         c.ic == IC::VortexPulseMix) && c.gas_constant != 1.0)
         throw InputError("ExtrudedIsentropicVortex requires Physics.GasConstant = 1 "
                          "(src/physics/euler/functions.cpp:107)");
 

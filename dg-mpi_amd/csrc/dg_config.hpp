@@ -41,7 +41,7 @@ enum class Basis { LagrangeHex = 0, LegendreHex = 1 };
 enum class Flux { LaxFriedrichs = 0, HLLC = 1, Roe = 2 };
 enum class Stepper { FE = 0, RK4 = 1 };
 enum class IC { Uniform = 0, Sphere = 1, IsentropicVortex = 2, ExtrudedIsentropicVortex = 3,
-                GaussianBump = 4, TaylorGreen = 5, Gaussian = 6 };
+                GaussianBump = 4, TaylorGreen = 5, Gaussian = 6, VortexPulseMix = 7 };
 
 struct CaseConfig {
     // [Physics] dim: 3 (hexahedra, 5 state variables) or 2 (quadrilaterals,

@@ -636,6 +636,37 @@ __device__ void ic_eval(const KIC& ic, const KGeom& g, const double* x, double t
         prim_to_cons(gam, 1. + A * exp(-r2 / (W * W)), Vv, D[5], U);
         break;
     }
+    case 7: {  // up to four z-extruded vortices and a Gaussian pressure pulse
+        // data [ub, vb, (xc, yc, beta) x 4, A, px, py, pz, w]: on rho = p = 1
+        // (T = 1, p / rho^gamma = 1) flowing at (ub, vb, 0), the Shu-vortex
+        // velocity and temperature perturbations of each vortex added (beta
+        // 0: absent), rho = T^(1 / (gamma - 1)), p = rho T; then an acoustic
+        // pulse p' = A exp(-r^2 / w^2), rho' = p' / gamma.  Periodic minimum
+        // images; no exact solution (t is ignored).
+        double du = 0., dv = 0., dT = 0.;
+        for (int k = 0; k < 4; ++k) {
+            const double vs = D[4 + 3 * k];
+            const double xr = wrap(x[0] - D[2 + 3 * k], g.period[0]), yr = wrap(x[1] - D[3 + 3 * k], g.period[1]);
+            const double r2 = xr * xr + yr * yr;
+            const double a = vs / (2.0 * M_PI) * exp(0.5 * (1.0 - r2));
+            du -= a * yr;
+            dv += a * xr;
+            dT -= (gam - 1.0) * vs * vs / (8.0 * gam * M_PI * M_PI) * exp(1.0 - r2);
+        }
+        const double T = 1.0 + dT;
+        double rho = pow(T, 1.0 / (gam - 1.0)), p = rho * T;
+        double r2 = 0.;
+        for (int d = 0; d < 3; ++d) {
+            const double xd = wrap(x[d] - D[15 + d], g.period[d]);
+            r2 += xd * xd;
+        }
+        const double pp = D[14] * exp(-r2 / (D[18] * D[18]));
+        rho += pp / gam;
+        p += pp;
+        const double u[3] = {D[0] + du, D[1] + dv, 0.};
+        prim_to_cons(gam, rho, u, p, U);
+        break;
+    }
     default: {  // Taylor-Green vortex at Mach D[0]
         const double Mn = D[0];
         const double p0 = 1. / (gam * Mn * Mn);

@@ -62,7 +62,7 @@ struct KGeom {
 // Initial condition / exact solution selector and data.
 struct KIC {
     int ic;                         // enum dg::IC
-    double data[10];
+    double data[24];
 };
 
 // Epilogue of the fused stage kernel.

@@ -15,33 +15,8 @@
 #include "dg_kernels.hpp"
 #include "dg_tables.hpp"
 
-struct dg_solver {
-    dg::CaseConfig cfg;
-    dg::KTables tab;
-    dg::KGeom geo;
-    dg::KIC ic;
-    int P = 0;
-    int ns = 5;                        // state variables: dim + 2
-    size_t ncoef = 0;                  // E * nb * ns
-    double dt = 0.0;                   // case time step (0 until known)
-    double time = 0.0;
-    int device = 0;
-    hipStream_t stream = nullptr;      // stream the handle launches on
-    hipStream_t own_stream = nullptr;  // stream created by dg_create
-    double* U0 = nullptr;              // solution (Solver::Uc)
-    double* Ua = nullptr;              // stage states
-    double* Ub = nullptr;
-    double* acc = nullptr;             // RK accumulator
-    double* work = nullptr;            // per-element reduction output
-    double* X = nullptr;               // mapped box: element node coordinates [E][8][3]
-    double* imm = nullptr;             // mapped box: inverse mass matrices [E][nb][nb]
-    unsigned* frec = nullptr;          // general path: face records [E][6][2]
-    double* nev = nullptr;             // general path: normal evaluation points
-    dg::KTables* dtab = nullptr;       // device copy of tab
-    std::vector<double> mesh_xyz;      // given mesh: node coordinates [n][dim]
-    std::vector<long> mesh_e2n;        // given mesh: element nodes [E][8] (2D: [E][4])
-    std::string err;
-};
+#include "dg_solver_impl.hpp"
+
 
 namespace {
 

@@ -568,7 +568,7 @@ KIC make_ic(const CaseConfig& c) {
     KIC k;
     std::memset(&k, 0, sizeof(k));
     k.ic = (int)c.ic;
-    for (size_t i = 0; i < c.ic_data.size() && i < 10; ++i) k.data[i] = c.ic_data[i];
+    for (size_t i = 0; i < c.ic_data.size() && i < 24; ++i) k.data[i] = c.ic_data[i];
     return k;
 }
 

@@ -36,7 +36,9 @@ EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get
            "dg_set_state", "dg_get_state", "dg_state_device_ptr", "dg_residual",
            "dg_apply_inv_mass", "dg_step", "dg_solve", "dg_l2_error", "dg_integrals",
            "dg_max_wavespeed", "dg_set_time", "dg_set_stream", "dg_batch_create", "dg_batch_step",
-           "dg_batch_destroy", "dg_batch_last_error"]
+           "dg_batch_destroy", "dg_batch_last_error", "dg_mlp_create", "dg_mlp_destroy", "dg_mlp_last_error",
+           "dg_mlp_num_params", "dg_mlp_get_params", "dg_mlp_set_params", "dg_mlp_set_scaling", "dg_mlp_forward",
+           "dg_mlp_gradient", "dg_mlp_train", "dg_closure_features", "dg_closure_target", "dg_closure_step"]
 
 
 class DGError(RuntimeError):
@@ -90,6 +92,18 @@ def lib():
             "dg_batch_create": [C.c_int, C.POINTER(vp), C.POINTER(vp)],
             "dg_batch_step": [vp, C.c_int],
             "dg_batch_destroy": [vp],
+            "dg_mlp_create": [C.c_ulonglong, C.POINTER(vp)],
+            "dg_mlp_destroy": [vp],
+            "dg_mlp_num_params": [C.POINTER(C.c_long)],
+            "dg_mlp_get_params": [vp, vp],
+            "dg_mlp_set_params": [vp, vp],
+            "dg_mlp_set_scaling": [vp, vp, vp, vp],
+            "dg_mlp_forward": [vp, vp, vp, C.c_long],
+            "dg_mlp_gradient": [vp, vp, vp, C.c_long, vp, dp],
+            "dg_mlp_train": [vp, vp, vp, C.c_long, vp, C.c_long, C.c_int, C.c_float, C.c_float, dp],
+            "dg_closure_features": [vp, vp, vp],
+            "dg_closure_target": [vp, vp, vp, vp, vp],
+            "dg_closure_step": [vp, vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_long)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -99,6 +113,8 @@ def lib():
         L.dg_last_error.restype = C.c_char_p
         L.dg_batch_last_error.argtypes = [vp]
         L.dg_batch_last_error.restype = C.c_char_p
+        L.dg_mlp_last_error.argtypes = [vp]
+        L.dg_mlp_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 

@@ -52,18 +52,20 @@ def synchronize():
 
 
 class Buffer:
-    """Device buffer of float64."""
+    """Device buffer of n elements of dtype (float64 by default)."""
 
-    def __init__(self, n):
+    def __init__(self, n, dtype=np.float64):
         self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = max(self.n, 1) * self.dtype.itemsize
         self.ptr = C.c_void_p()
-        check(rt().hipMalloc(C.byref(self.ptr), max(self.n, 1) * 8), "hipMalloc")
-        check(rt().hipMemset(self.ptr, 0, max(self.n, 1) * 8), "hipMemset")
+        check(rt().hipMalloc(C.byref(self.ptr), self.nbytes), "hipMalloc")
+        check(rt().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
     @classmethod
-    def from_array(cls, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        b = cls(a.size)
+    def from_array(cls, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        b = cls(a.size, dtype)
         b.upload(a)
         return b
 
@@ -72,13 +74,15 @@ class Buffer:
         return self.ptr.value
 
     def upload(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.size == self.n
-        check(rt().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.n * 8, 1), "hipMemcpy H2D")
+        check(rt().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.n * self.dtype.itemsize, 1),
+              "hipMemcpy H2D")
 
     def download(self, shape=None):
-        a = np.empty(self.n, dtype=np.float64)
-        check(rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), self.ptr, self.n * 8, 2), "hipMemcpy D2H")
+        a = np.empty(self.n, dtype=self.dtype)
+        check(rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), self.ptr, self.n * self.dtype.itemsize, 2),
+              "hipMemcpy D2H")
         return a.reshape(shape) if shape is not None else a
 
     def free(self):

@@ -62,7 +62,8 @@ typedef struct {
     int basis;               /* 0 LagrangeHex, 1 LegendreHex                 */
     int ic;                  /* 0 Uniform, 1 Sphere, 2 IsentropicVortex,
                                 3 ExtrudedIsentropicVortex, 4 GaussianBump,
-                                5 TaylorGreen, 6 Gaussian (2D)               */
+                                5 TaylorGreen, 6 Gaussian (2D),
+                                7 VortexPulseMix                             */
     int dim;                 /* 3: hexahedra, 5 state variables; 2:
                                 quadrilaterals, 4 state variables (rho,
                                 rho u, rho v, rho E), nelem[2] = 1           */
@@ -200,6 +201,50 @@ int dg_batch_destroy(dg_batch* b);
 /* Last error of the batch (NULL: of the last failed dg_batch_create on this
  * thread).  Never returns NULL. */
 const char* dg_batch_last_error(const dg_batch* b);
+
+/* Learned closure for coarse runs (no reference counterpart).  A per-element
+ * MLP -- 70 features -> 128 -> 128 -> 40, GELU, BF16 weights with FP32
+ * accumulation, FP32 master weights trained with Adam by this library's
+ * own forward / backward kernels -- corrects the tendency of a p = 1
+ * LegendreHex periodic box: dU/dt = M^-1 R(U) + C(U).  Features of an
+ * element: its 40 coefficients, then the jumps of the face-mean state
+ * (neighbour minus own) across its x-, x+, y-, y+, z-, z+ faces (5 each).
+ * Targets: with a finer (p >= 2, Legendre) solver on the same mesh, Uc =
+ * the p = 1 truncation of the fine state (its L2 projection) and
+ * Y = trunc(M_f^-1 R_f(U_f)) - M_c^-1 R_c(Uc).  The network sees features
+ * normalised as (x - x_mean) * x_scale and predicts Y / y_scale.  Device
+ * arrays are caller-owned: X [rows][70], Y [rows][40] float, states double.
+ * Parameters (dg_mlp_num_params floats): W1 [70][128], b1, W2 [128][128],
+ * b2, W3 [128][40], b3. */
+typedef struct dg_mlp dg_mlp;
+int dg_mlp_create(unsigned long long seed, dg_mlp** out);
+int dg_mlp_destroy(dg_mlp* m);
+const char* dg_mlp_last_error(const dg_mlp* m);
+int dg_mlp_num_params(long* n);
+int dg_mlp_get_params(const dg_mlp* m, float* host);
+int dg_mlp_set_params(dg_mlp* m, const float* host);
+int dg_mlp_set_scaling(dg_mlp* m, const float* x_mean, const float* x_scale, const float* y_scale);
+/* Y_dev = C(X_dev), de-normalised */
+int dg_mlp_forward(dg_mlp* m, const float* X_dev, float* Y_dev, long rows);
+/* gradient (host, num_params floats) and loss of the mean squared error
+ * in normalised target units over all rows */
+int dg_mlp_gradient(dg_mlp* m, const float* X_dev, const float* Y_dev, long rows, float* grad_host, double* loss);
+/* Adam steps (lr; beta 0.9 / 0.999) over consecutive mini-batches of
+ * `batch` rows named by idx_dev[0 .. nidx), the features jittered by
+ * Gaussian noise of standard deviation `noise` (normalised units; 0: none);
+ * mean loss of those steps */
+int dg_mlp_train(dg_mlp* m, const float* X_dev, const float* Y_dev, long nrows, const int* idx_dev, long nidx,
+                 int batch, float lr, float noise, double* mean_loss);
+/* features of the p = 1 state U_dev of solver s */
+int dg_closure_features(dg_solver* s, const double* U_dev, float* X_dev);
+/* coarse state Uc_dev and target Y_dev of the fine state Uf_dev (uses the
+ * handles' stage registers as scratch) */
+int dg_closure_target(dg_solver* fine, const double* Uf_dev, dg_solver* coarse, double* Uc_dev, float* Y_dev);
+/* nsteps RK4 steps of size dt (<= 0: the case's time step) of the p = 1
+ * solver s with the corrected tendency; elements whose correction has a
+ * 2-norm above `bound` get none (the plain residual), counted over all
+ * stages in *n_fallback */
+int dg_closure_step(dg_solver* s, dg_mlp* m, double dt, double bound, int nsteps, long* n_fallback);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,170 @@
+"""The learned-closure kernels against the CPU oracle: the network's
+forward pass and gradient (oracle/mlp.py, float64 with the same bfloat16
+weight rounding; the GPU accumulates in FP32, bar 1e-4 relative per
+parameter block), the coarse-state features and the training targets
+(oracle/oracle.py residuals; FP32 outputs, bar 1e-6 relative), closure
+stepping with a network whose output is zero against plain RK4 (1e-13),
+and a trained network's held-out gain."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import dg_amd
+import mlp as MO
+import oracle as O
+from dg_amd import closure, hip
+
+pytestmark = pytest.mark.gpu
+
+VORTEX = ("ExtrudedIsentropicVortex", (1.0, 0.5, 0.3, 1.0, 3.0))
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    assert hip.device_count() > 0, "GPU test needs a GPU"
+
+
+def rand_net(seed=0):
+    rng = np.random.default_rng(seed)
+    p = np.concatenate([rng.normal(0, 0.2, int(np.prod(s))) for _, s in MO.SHAPES]).astype(np.float32)
+    xm = rng.normal(0, 0.1, MO.NF).astype(np.float32)
+    xs = rng.uniform(0.5, 2.0, MO.NF).astype(np.float32)
+    ys = rng.uniform(0.5, 2.0, MO.NO).astype(np.float32)
+    return p, xm, xs, ys
+
+
+def test_mlp_forward_and_gradient_against_the_oracle(gpu):
+    p, xm, xs, ys = rand_net()
+    rng = np.random.default_rng(3)
+    rows = 300
+    X = rng.normal(0, 1, (rows, MO.NF)).astype(np.float32)
+    Y = rng.normal(0, 1, (rows, MO.NO)).astype(np.float32)
+    m = closure.MLP(0)
+    m.set_params(p)
+    m.set_scaling(xm, xs, ys)
+    Xd, Yd = hip.Buffer.from_array(X, np.float32), hip.Buffer.from_array(Y, np.float32)
+    out = m.forward(Xd, rows).download((rows, MO.NO))
+    ref, _ = MO.forward(p, X, xm.astype(np.float64), xs.astype(np.float64))
+    ref = ref * ys
+    assert np.max(np.abs(out - ref)) < 2e-5 * np.max(np.abs(ref))
+    g, loss = m.gradient(Xd, Yd, rows)
+    lref, gref = MO.loss_and_grad(p, X, Y, xm.astype(np.float64), xs.astype(np.float64), ys.astype(np.float64))
+    assert abs(loss - lref) < 1e-5 * lref
+    o = 0
+    for name, shp in MO.SHAPES:
+        n = int(np.prod(shp))
+        a, b = g[o:o + n], gref[o:o + n]
+        e = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(f"{name}: max rel. diff {e:.2e}", flush=True)
+        assert e < 1e-4, name
+        o += n
+    # one Adam step moves the loss down on this batch
+    idx = hip.Buffer.from_array(np.arange(rows, dtype=np.int32), np.int32)
+    l0 = m.train(Xd, Yd, rows, idx, rows, 1e-3)
+    _, l1 = m.gradient(Xd, Yd, rows)
+    assert l1 < l0
+    m.close()
+
+
+def p1_faces_ref(U, ne):
+    """face-mean jumps of a p = 1 Legendre state from the oracle's basis"""
+    nx, ny, nz = ne
+    x, w = O.gl_rule(3)
+    phi = np.array([O.basis_1d("LegendreHex", 1, v)[0] for v in (-1.0, 1.0)])   # [side][function]
+    mean = 0.5 * (w @ np.array([O.basis_1d("LegendreHex", 1, v)[0] for v in x]))  # (1/2) int phi_j
+    Ub = U.reshape(nz, ny, nx, 2, 2, 2, 5)                           # k, j, i index order of b = i + 2j + 4k
+    out = np.zeros((nz, ny, nx, 6, 5))
+
+    def fm(u, d, side):
+        # u [..., k, j, i, s]
+        if d == 0:
+            return np.einsum("...kjis,i,j,k->...s", u, phi[side], mean, mean)
+        if d == 1:
+            return np.einsum("...kjis,i,j,k->...s", u, mean, phi[side], mean)
+        return np.einsum("...kjis,i,j,k->...s", u, mean, mean, phi[side])
+
+    for d in range(3):
+        for side in range(2):
+            shift = 1 if side else -1
+            axis = 2 - d                                               # array axes: z, y, x
+            nb = np.roll(Ub, -shift, axis=axis)
+            out[:, :, :, d * 2 + side] = fm(nb, d, 1 - side) - fm(Ub, d, side)
+    return out.reshape(nx * ny * nz, 30)
+
+
+def test_features_against_the_oracle(gpu):
+    ne = (4, 3, 5)
+    with dg_amd.Solver(dg_amd.case_toml(order=1, basis="LegendreHex", nelem=ne, timestep=0.01, ic=VORTEX[0],
+                                        ic_data=VORTEX[1])) as g:
+        g.init_state_from_fcn()
+        U = g.get_state()
+        X = closure.features(g, g.state_device_ptr()).download((U.shape[0], MO.NF))
+        assert np.allclose(X[:, :40], U.reshape(-1, 40).astype(np.float32), rtol=0, atol=0)
+        ref = p1_faces_ref(U, ne)
+        assert np.max(np.abs(X[:, 40:] - ref)) < 1e-6 * max(np.max(np.abs(ref)), 1e-3)
+
+
+def test_targets_against_the_oracle(gpu):
+    ne = (3, 4, 3)
+    fine_t = dg_amd.case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=ne, timestep=0.01, ic=VORTEX[0],
+                              ic_data=VORTEX[1])
+    coarse_t = fine_t.replace("order = 3", "order = 1")
+    with dg_amd.Solver(fine_t) as f, dg_amd.Solver(coarse_t) as c:
+        f.init_state_from_fcn()
+        Uf = f.get_state()
+        E = Uf.shape[0]
+        Uc = hip.Buffer(E * 40)
+        Y = closure.target(f, f.state_device_ptr(), c, Uc).download((E, 40))
+        of = O.Solver(O.make_params(order=3, basis="LegendreHex", flux="Roe", nelem=ne, lo=(-5, -5, -5),
+                                    hi=(5, 5, 5), ic=VORTEX[0], ic_data=VORTEX[1]))
+        oc = O.Solver(O.make_params(order=1, basis="LegendreHex", flux="Roe", nelem=ne, lo=(-5, -5, -5),
+                                    hi=(5, 5, 5), ic=VORTEX[0], ic_data=VORTEX[1]))
+        keep = [i + 4 * j + 16 * k for k in range(2) for j in range(2) for i in range(2)]
+        trunc = Uf[:, keep, :]
+        assert np.array_equal(Uc.download((E, 8, 5)), trunc)
+        Df = of.inv_mass(1.0, of.residual(Uf))[:, keep, :]
+        Dc = oc.inv_mass(1.0, oc.residual(trunc))
+        ref = (Df - Dc).reshape(E, 40)
+        assert np.max(np.abs(Y - ref)) < 1e-6 * np.max(np.abs(ref))
+
+
+def test_zero_network_steps_like_plain_rk4(gpu):
+    t = dg_amd.case_toml(order=1, basis="LegendreHex", flux="HLLC", nelem=(6, 5, 4), timestep=0.02, ic=VORTEX[0],
+                         ic_data=VORTEX[1])
+    m = closure.MLP(1)
+    m.set_params(np.zeros(closure.num_params(), np.float32))
+    with dg_amd.Solver(t) as a, dg_amd.Solver(t) as b:
+        a.init_state_from_fcn()
+        b.init_state_from_fcn()
+        nfb = closure.step(a, m, 1e30, 5)
+        b.take_time_step(0.0, 5)
+        assert nfb == 0
+        assert a.time == b.time
+        Ua, Ub = a.get_state(), b.get_state()
+        assert np.max(np.abs(Ua - Ub)) < 1e-13 * np.max(np.abs(Ub))
+        # a bound of 0 turns every nonzero correction off: the same result
+    m.close()
+
+
+def test_trained_network_gain_and_fallback(gpu):
+    """What the trained network gains (a priori): on held-out snapshots of
+    fine runs it removes most of the coarse tendency's error (mean squared
+    error below half that of the zero correction).  In rollouts the
+    correction feeds back on its own errors; the norm-bound fallback keeps
+    the corrected run finite and at the plain p = 1 error (within 5 %)."""
+    n = 12
+    X, Y, rows = closure.generate([2000, 2001, 2002, 2003], n, fine_order=3, snapshots=6, every=6)
+    m, hist = closure.fit(X, Y, rows, epochs=4, batch=2048, lr=2e-3, seed=0)
+    print("losses", hist, flush=True)
+    assert hist[-1] < hist[0]
+    Xv, Yv, rv = closure.generate([7, 8], n, fine_order=3, snapshots=6, every=6)
+    mse_net, mse_zero = closure.a_priori(m, Xv, Yv, rv)
+    print(f"held-out MSE: network {mse_net:.3e}, zero correction {mse_zero:.3e}", flush=True)
+    assert mse_net < 0.5 * mse_zero
+    ev = closure.evaluate(m, [7], n, 0.3, fine_order=3, bounds=(2.0,), compare_orders=())
+    r = ev[0]
+    print(r, flush=True)
+    assert np.isfinite(r["closure_b2_rho_error"])
+    assert r["closure_b2_rho_error"] < 1.05 * r["p1_rho_error"]
+    m.close()
