@@ -206,29 +206,31 @@ def batch_bench(args):
 
 def closure_bench(args):
     """Learned closure (coarse p = 1 on a 32^3 periodic box): training rows
-    from FP64 p = 4 runs of seeded vortex / acoustic-pulse mixtures, the
-    network trained by the library's own kernels, then held-out seeds run
-    to t = 0.5 with plain p = 1, p = 1 with the closure and plain p = 3,
-    each compared with the p = 4 run projected to p = 1.  One JSON line."""
+    along coarse runs beside FP64 p = 4 runs of seeded vortex /
+    acoustic-pulse mixtures (two rollout-training rounds, relaxation time 4
+    coarse steps), the network trained by the library's own kernels; then
+    held-out seeds run to t = 0.5 with plain p = 1, p = 1 with the closure
+    (no bound, and the fallback at 2x the targets' RMS element norm), plain
+    p = 2 and p = 3, each compared with the p = 4 run projected to p = 1.
+    One JSON line."""
     import numpy as np
     from dg_amd import closure, hip
 
     n = args.closure
     t0 = time.perf_counter()
     train_seeds = list(range(1000, 1000 + args.closure_seeds))
-    X, Y, rows = closure.generate(train_seeds, n, fine_order=4, snapshots=12, every=6)
-    t_gen = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    m, hist = closure.fit(X, Y, rows, epochs=args.closure_epochs, batch=4096, lr=2e-3, seed=0)
+    log = lambda s: print(s, file=sys.stderr, flush=True)     # progress lines
+    m, hist, rows = closure.train_rollouts(train_seeds, n, fine_order=4, steps=24, tau_steps=4.0,
+                                           epochs=args.closure_epochs, bound=2.0, log=log)
     t_fit = time.perf_counter() - t0
-    X.free()
-    Y.free()
-    # a priori: held-out snapshots, network vs the zero correction
-    Xv, Yv, rv = closure.generate([1, 2, 3, 4], n, fine_order=4, snapshots=12, every=6)
+    t_gen = None
+    # a priori: held-out rows along plain coarse runs, network vs the zero correction
+    Xv, Yv, rv = closure.generate_rollout([1, 2, 3, 4], n, fine_order=4, steps=24, tau_steps=4.0)
     mse_net, mse_zero = closure.a_priori(m, Xv, Yv, rv)
     Xv.free()
     Yv.free()
-    bounds, relax = (None, 2.0), (1.0, 0.5, 0.25)
+    bounds, relax = (None, 2.0), (1.0,)
+    log("evaluating held-out seeds")
     ev = closure.evaluate(m, [1, 2, 3, 4], n, 0.5, fine_order=4, bounds=bounds, relax=relax, compare_orders=(2, 3))
     mean = lambda k: float(np.mean([r[k] for r in ev]))
     keys = [("closure" if b is None else f"closure_b{b:g}") + ("" if a == 1.0 else f"_r{a:g}")
@@ -237,7 +239,7 @@ def closure_bench(args):
         "metric": "coarse p=1 density L2 error vs the p=4 run projected to p=1, held-out seeds, t = 0.5",
         "config": {"workload": f"closure_{n}^3_p1_mlp70-128-128-40_bf16", "train_seeds": len(train_seeds),
                    "rows": int(rows), "epochs": args.closure_epochs},
-        "generate_s": t_gen, "fit_s": t_fit, "loss_history": hist,
+        "train_s": t_fit, "loss_history": hist,
         "target_rms_element_norm": m.target_rms,
         "a_priori_heldout_mse": {"network": mse_net, "zero_correction": mse_zero,
                                  "fraction_removed": 1.0 - mse_net / mse_zero},
@@ -276,8 +278,8 @@ def main():
     ap.add_argument("--seed", type=int, default=7, help=argparse.SUPPRESS)
     # learned closure: coarse mesh edge, training seeds, epochs
     ap.add_argument("--closure", type=int, default=0, help=argparse.SUPPRESS)
-    ap.add_argument("--closure-seeds", type=int, default=24, help=argparse.SUPPRESS)
-    ap.add_argument("--closure-epochs", type=int, default=8, help=argparse.SUPPRESS)
+    ap.add_argument("--closure-seeds", type=int, default=16, help=argparse.SUPPRESS)
+    ap.add_argument("--closure-epochs", type=int, default=6, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

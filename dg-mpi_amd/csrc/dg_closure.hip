@@ -275,6 +275,21 @@ __global__ void descale_kernel(const float* __restrict__ O, const float* __restr
     if (t < n) C[t] = O[t] * ys[t % NO];
 }
 
+// Y = trunc(Df) - Dc + (trunc(Uf) - Uc) / tau, for a coarse state Uc of
+// its own (a coarse run beside the fine one): the tendency mismatch at Uc
+// plus a relaxation of Uc toward the fine state's projection
+__global__ void target_at_kernel(const double* __restrict__ Df, const double* __restrict__ Dc,
+                                 const double* __restrict__ Uf, const double* __restrict__ Uc, double rtau,
+                                 float* __restrict__ Y, int Pf, long E) {
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= E * 40) return;
+    const long e = t / 40;
+    const int r = (int)(t % 40), b = r / 5, s = r % 5;
+    const int i = b & 1, j = (b >> 1) & 1, k = b >> 2, n = Pf + 1;
+    const long f = (e * n * n * n + i + n * j + n * n * k) * 5 + s;
+    Y[t] = (float)(Df[f] - Dc[t] + (Uf[f] - Uc[t]) * rtau);
+}
+
 // fallback: elements whose correction norm exceeds the bound get none
 __global__ void bound_kernel(float* __restrict__ C, double bound, long E, unsigned* __restrict__ count) {
     const long e = (long)blockIdx.x * TB + threadIdx.x;
@@ -648,6 +663,34 @@ int dg_closure_target(dg_solver* fine, const double* Uf_dev, dg_solver* coarse, 
         hcheck(hipStreamSynchronize(coarse->stream), "hipStreamSynchronize");
         hipLaunchKernelGGL(target_kernel, dim3(nblk(E * 40)), dim3(TB), 0, fine->stream, fine->Ub, coarse->Ub, Y_dev,
                            fine->P, E);
+        hcheck(hipGetLastError(), "target kernel");
+        hcheck(hipStreamSynchronize(fine->stream), "hipStreamSynchronize");
+    });
+}
+
+int dg_closure_target_at(dg_solver* fine, const double* Uf_dev, dg_solver* coarse, const double* Uc_dev, double tau,
+                         float* Y_dev) {
+    return mlp_guard(g_mlp_error, [&] {
+        check_box(fine, "fine solver", 0);
+        check_box(coarse, "coarse solver", 1);
+        need_ptr(Uf_dev, "Uf");
+        need_ptr(Uc_dev, "Uc");
+        need_ptr(Y_dev, "Y");
+        if (fine->P < 2) throw std::invalid_argument("the fine solver's order must exceed 1");
+        if (!(tau > 0.0)) throw std::invalid_argument("tau must be > 0");
+        for (int d = 0; d < 3; ++d)
+            if (fine->geo.ne[d] != coarse->geo.ne[d] || fine->geo.h[d] != coarse->geo.h[d] ||
+                fine->geo.lo[d] != coarse->geo.lo[d])
+                throw std::invalid_argument("fine and coarse solvers must share the mesh");
+        if (fine->device != coarse->device) throw std::invalid_argument("the solvers live on different devices");
+        hcheck(hipSetDevice(fine->device), "hipSetDevice");
+        const long E = fine->geo.E;
+        tendency(fine, Uf_dev, fine->Ub);
+        hcheck(hipStreamSynchronize(fine->stream), "hipStreamSynchronize");
+        tendency(coarse, Uc_dev, coarse->Ub);
+        hcheck(hipStreamSynchronize(coarse->stream), "hipStreamSynchronize");
+        hipLaunchKernelGGL(target_at_kernel, dim3(nblk(E * 40)), dim3(TB), 0, fine->stream, fine->Ub, coarse->Ub,
+                           Uf_dev, Uc_dev, 1.0 / tau, Y_dev, fine->P, E);
         hcheck(hipGetLastError(), "target kernel");
         hcheck(hipStreamSynchronize(fine->stream), "hipStreamSynchronize");
     });

@@ -38,7 +38,8 @@ EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get
            "dg_max_wavespeed", "dg_set_time", "dg_set_stream", "dg_batch_create", "dg_batch_step",
            "dg_batch_destroy", "dg_batch_last_error", "dg_mlp_create", "dg_mlp_destroy", "dg_mlp_last_error",
            "dg_mlp_num_params", "dg_mlp_get_params", "dg_mlp_set_params", "dg_mlp_set_scaling", "dg_mlp_forward",
-           "dg_mlp_gradient", "dg_mlp_train", "dg_closure_features", "dg_closure_target", "dg_closure_step"]
+           "dg_mlp_gradient", "dg_mlp_train", "dg_closure_features", "dg_closure_target", "dg_closure_target_at",
+           "dg_closure_step"]
 
 
 class DGError(RuntimeError):
@@ -103,6 +104,7 @@ def lib():
             "dg_mlp_train": [vp, vp, vp, C.c_long, vp, C.c_long, C.c_int, C.c_float, C.c_float, dp],
             "dg_closure_features": [vp, vp, vp],
             "dg_closure_target": [vp, vp, vp, vp, vp],
+            "dg_closure_target_at": [vp, vp, vp, vp, C.c_double, vp],
             "dg_closure_step": [vp, vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_long)],
         }
         for name, args in sig.items():

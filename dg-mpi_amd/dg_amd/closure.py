@@ -96,6 +96,70 @@ def target(fine, Uf_ptr, coarse, Uc, Y=None):
     return Y
 
 
+def target_at(fine, Uf_ptr, coarse, Uc_ptr, tau, Y):
+    _check(lib().dg_closure_target_at(fine._h, C.c_void_p(Uf_ptr), coarse._h, C.c_void_p(Uc_ptr), tau, Y.addr))
+    return Y
+
+
+def generate_rollout(seeds, n, fine_order=4, steps=24, tau_steps=4.0, model=None, bound=None, out=None, log=None):
+    """Training rows along coarse runs beside fine ones: per seed the fine
+    run and a coarse p = 1 run (plain, or with `model` and its fallback
+    bound) advance together, the coarse step an integer number of fine
+    steps; after each coarse step, the features of the coarse state and the
+    target of dg_closure_target_at with tau = tau_steps coarse steps.
+    out = (X, Y, first_row): write into existing buffers from that row."""
+    from . import Solver
+    E = n ** 3
+    rows = len(seeds) * steps * E
+    if out is None:
+        X = hip.Buffer(rows * NF, np.float32)
+        Y = hip.Buffer(rows * NO, np.float32)
+        r = 0
+    else:
+        X, Y, r = out
+    for sd in seeds:
+        data = mix_ic(np.random.default_rng(sd))
+        with Solver(case(fine_order, n, data, CFL[fine_order])) as f, Solver(case(1, n, data, CFL[1])) as c:
+            f.init_state_from_fcn()
+            c.init_state_from_fcn()
+            dtc = c.timestep
+            mult = int(np.ceil(dtc / f.timestep))
+            dtf = dtc / mult
+            for k in range(steps):
+                f.take_time_step(dtf, mult)
+                if model is None:
+                    c.take_time_step(dtc, 1)
+                else:
+                    step(c, model, 1e30 if bound is None else bound * model.target_rms, 1, dtc)
+                target_at(f, f.state_device_ptr(), c, c.state_device_ptr(), tau_steps * dtc,
+                          _At(Y.addr + r * NO * 4))
+                features(c, c.state_device_ptr(), _At(X.addr + r * NF * 4))
+                r += E
+        if log:
+            log(f"rollout rows: seed {sd} done ({r} rows)")
+    hip.synchronize()
+    return X, Y, rows
+
+
+def train_rollouts(seeds, n, fine_order=4, steps=24, tau_steps=4.0, epochs=6, bound=2.0, log=None):
+    """Two rounds of rollout training (DAgger, Ross et al. 2011): rows
+    along plain coarse runs, a first fit; rows along coarse runs corrected
+    by that network (fallback bound in target-RMS units), a second fit on
+    both sets.  Returns the second network and the loss histories."""
+    E = n ** 3
+    per = len(seeds) * steps * E
+    X = hip.Buffer(2 * per * NF, np.float32)
+    Y = hip.Buffer(2 * per * NO, np.float32)
+    generate_rollout(seeds, n, fine_order, steps, tau_steps, out=(X, Y, 0), log=log)
+    m1, h1 = fit(X, Y, per, epochs=epochs, batch=4096, lr=2e-3, seed=0, log=log)
+    generate_rollout(seeds, n, fine_order, steps, tau_steps, model=m1, bound=bound, out=(X, Y, per), log=log)
+    m1.close()
+    m2, h2 = fit(X, Y, 2 * per, epochs=epochs, batch=4096, lr=2e-3, seed=1, log=log)
+    X.free()
+    Y.free()
+    return m2, (h1, h2), 2 * per
+
+
 def step(solver, mlp, bound, nsteps, dt=0.0):
     n = C.c_long()
     _check(lib().dg_closure_step(solver._h, mlp._h, dt, bound, nsteps, C.byref(n)))
@@ -166,9 +230,12 @@ def scaling_from(X, Y, rows, sample=200000, seed=0):
     """feature mean and 1/std, target std, and the RMS of the targets'
     per-element 2-norm (the unit of the fallback bound)"""
     rng = np.random.default_rng(seed)
-    Xh = X.download((rows, NF))
-    Yh = Y.download((rows, NO))
-    pick = rng.choice(rows, min(sample, rows), replace=False)
+    # 16 blocks of rows spread over the first `rows` rows (the buffers may hold more)
+    blk = max(1, min(rows, sample) // 16)
+    starts = np.linspace(0, rows - blk, 16).astype(np.int64)
+    Xh = np.concatenate([X.download_range(s0 * NF, blk * NF) for s0 in starts]).reshape(-1, NF)
+    Yh = np.concatenate([Y.download_range(s0 * NO, blk * NO) for s0 in starts]).reshape(-1, NO)
+    pick = np.arange(len(Xh))
     xm = Xh[pick].mean(0)
     xs = 1.0 / np.maximum(Xh[pick].std(0), 1e-6)
     ys = np.maximum(Yh[pick].std(0), 1e-8)
@@ -209,7 +276,7 @@ def a_priori(m, X, Y, rows):
     the zero correction (the plain coarse tendency): what the network
     removes of the coarse tendency's error"""
     _, loss = m.gradient(X, Y, rows)
-    Yh = Y.download((rows, NO)).astype(np.float64) / m.scaling[2].astype(np.float64)
+    Yh = Y.download((rows, NO), rows * NO).astype(np.float64) / m.scaling[2].astype(np.float64)
     return loss, float(np.mean(Yh * Yh))
 
 

@@ -79,11 +79,23 @@ class Buffer:
         check(rt().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.n * self.dtype.itemsize, 1),
               "hipMemcpy H2D")
 
-    def download(self, shape=None):
-        a = np.empty(self.n, dtype=self.dtype)
-        check(rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), self.ptr, self.n * self.dtype.itemsize, 2),
+    def download(self, shape=None, count=None):
+        """the first `count` elements (default all), optionally reshaped"""
+        n = self.n if count is None else int(count)
+        assert 0 <= n <= self.n
+        a = np.empty(n, dtype=self.dtype)
+        check(rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), self.ptr, n * self.dtype.itemsize, 2),
               "hipMemcpy D2H")
         return a.reshape(shape) if shape is not None else a
+
+    def download_range(self, start, count):
+        """elements start .. start + count - 1"""
+        start, count = int(start), int(count)
+        assert 0 <= start and start + count <= self.n
+        a = np.empty(count, dtype=self.dtype)
+        src = C.c_void_p(self.ptr.value + start * self.dtype.itemsize)
+        check(rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), src, count * self.dtype.itemsize, 2), "hipMemcpy D2H")
+        return a
 
     def free(self):
         if self.ptr:

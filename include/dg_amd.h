@@ -240,6 +240,11 @@ int dg_closure_features(dg_solver* s, const double* U_dev, float* X_dev);
 /* coarse state Uc_dev and target Y_dev of the fine state Uf_dev (uses the
  * handles' stage registers as scratch) */
 int dg_closure_target(dg_solver* fine, const double* Uf_dev, dg_solver* coarse, double* Uc_dev, float* Y_dev);
+/* target for a coarse state of its own (a coarse run beside the fine one):
+ * Y = trunc(M_f^-1 R_f(U_f)) - M_c^-1 R_c(Uc) + (trunc(U_f) - Uc) / tau,
+ * the tendency mismatch at Uc plus a relaxation toward the fine state */
+int dg_closure_target_at(dg_solver* fine, const double* Uf_dev, dg_solver* coarse, const double* Uc_dev, double tau,
+                         float* Y_dev);
 /* nsteps RK4 steps of size dt (<= 0: the case's time step) of the p = 1
  * solver s with the corrected tendency; elements whose correction has a
  * 2-norm above `bound` get none (the plain residual), counted over all
