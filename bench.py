@@ -282,7 +282,7 @@ def main():
     ap.add_argument("--closure-epochs", type=int, default=6, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the path runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path is measured at one GPU's real shape)", file=sys.stderr)
         sys.exit(2)
 

@@ -50,7 +50,7 @@ struct CaseConfig {
     // [Numerics] (src/numerics/numerics_data.cpp:229-300)
     int order = 0;
     Basis basis = Basis::LagrangeHex;
-    // precision = "FP64" (default) | "FP32" (a key of this build, no
+    // precision = "FP64" (default) | "FP32" (a new key, no
     // reference counterpart): FP32 face and volume work in the stage
     // kernel, FP64 RK registers, M^-1 and reductions; 3D periodic box only
     bool fp32 = false;

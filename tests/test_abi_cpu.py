@@ -230,7 +230,7 @@ def test_2d_mesh_input_is_checked_before_the_device():
 
 
 def test_precision_key():
-    """Numerics.precision (this build's key): FP64 (default) or FP32, the
+    """Numerics.precision (a new key): FP64 (default) or FP32, the
     FP32 form on the affine 3D box only."""
     from dg_amd import hip
     base = dg_amd.case_toml(order=2, nelem=(2, 2, 2), timestep=0.01, precision="FP32")
