@@ -117,6 +117,7 @@ def io_bench(s, path):
 def batch_bench(args):
     """Batched uneven cases: 256 independent periodic vortex problems (seed
     args.seed) with n^3 elements, n uniform in 8..32, order p in {1..4},
+    vortex centre (x, y) in [-2, 2]^2 (the vortex is z-extruded),
     vortex strength beta in [1, 5], free-stream Mach number in [0.2, 0.8]
     along (1, 1, 0) (rho = p = 1), Legendre, Roe, RK4 at CFL 0.1, 0.1, 0.06,
     0.04 for p = 1..4 (the dt = CFL h / max(|u| + c) rule is stable to these
@@ -138,8 +139,9 @@ def batch_bench(args):
         p = int(rng.integers(1, 5))
         beta = float(rng.uniform(1.0, 5.0))
         mach = float(rng.uniform(0.2, 0.8))
+        xc, yc = (float(v) for v in rng.uniform(-2.0, 2.0, 2))
         u = mach * np.sqrt(1.4) / np.sqrt(2.0)
-        cases.append((n, p, beta, mach, (1.0, u, u, 1.0, beta)))
+        cases.append((n, p, beta, mach, (1.0, u, u, 1.0, beta, xc, yc)))
 
     def make(precision):
         out = []

@@ -604,13 +604,14 @@ __device__ void ic_eval(const KIC& ic, const KGeom& g, const double* x, double t
         U[4] = p / (gam - 1.0) + 0.5 * rKE;
         break;
     }
-    case 3: {  // z-extruded 2D vortex, advected with periodic minimum image
+    case 3: {  // z-extruded 2D vortex, advected with periodic minimum image;
+               // D[5], D[6]: initial centre (0 when not given)
         const double rhob = D[0], ub = D[1], vb = D[2], pb = D[3], vs = D[4];
         const double Rg = g.R;
         const double Tb = pb / (rhob * Rg);
         const double s = pb / (pow(rhob, gam));
-        const double xr = wrap(x[0] - ub * t, g.period[0]);
-        const double yr = wrap(x[1] - vb * t, g.period[1]);
+        const double xr = wrap((x[0] - D[5]) - ub * t, g.period[0]);
+        const double yr = wrap((x[1] - D[6]) - vb * t, g.period[1]);
         const double r = sqrt(xr * xr + yr * yr);
         const double dU = vs / (2.0 * M_PI) * exp(0.5 * (1.0 - r * r));
         const double du = dU * -1.0 * yr, dv = dU * xr;

@@ -615,7 +615,9 @@ static void prim_to_cons(double gam, double rho, const double* u, double p, doub
  *   VORTEX3D         src/physics/euler/functions.cpp:147-191 (t ignored)
  *   VORTEX_EXTRUDED  the 2D formula src/physics/euler/functions.cpp:94-145
  *                    applied to (x, y), extruded along z (new); the advected
- *                    core is taken at its minimum periodic image (new).
+ *                    core is taken at its minimum periodic image (new);
+ *                    optional data[5], data[6]: the core's initial centre
+ *                    (default the origin, as the reference's formula).
  *   GAUSSIAN_BUMP    new: rho = 1 + A exp(-(r/W)^2) carried by uniform V, p0.
  *   TAYLOR_GREEN     new: inviscid TGV at Mach M, p0 = 1/(gamma M^2); no
  *                    exact solution, t ignored. */
@@ -655,8 +657,8 @@ void orc_ic_eval(const orc_params* prm, const double* x, double t, double* U) {
         if (Rg != 1.0) die("extruded isentropic vortex requires GasConstant = 1");
         double Tb = pb / (rhob * Rg);
         double s = pb / (pow(rhob, gam));
-        double xr = wrap(x[0] - ub * t, prm->hi[0] - prm->lo[0]);
-        double yr = wrap(x[1] - vb * t, prm->hi[1] - prm->lo[1]);
+        double xr = wrap((x[0] - D[5]) - ub * t, prm->hi[0] - prm->lo[0]);
+        double yr = wrap((x[1] - D[6]) - vb * t, prm->hi[1] - prm->lo[1]);
         double r = sqrt(pow(xr, 2) + pow(yr, 2));
         double dU = vs / (2.0 * M_PI) * exp(0.5 * (1.0 - pow(r, 2)));
         double du = dU * -1.0 * yr, dv = dU * xr;

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 CASES = [
     # order, basis, flux, nelem, (rho, u, v, p, beta), timestep (None: CFL 0.1), precision
     (1, "LegendreHex", "Roe", (5, 4, 3), (1.0, 0.5, 0.3, 1.0, 2.0), None, None),
-    (2, "LagrangeHex", "HLLC", (3, 3, 3), (1.0, 1.0, 1.0, 1.0, 5.0), 0.01, None),
+    (2, "LagrangeHex", "HLLC", (3, 3, 3), (1.0, 1.0, 1.0, 1.0, 5.0, 1.3, -0.7), 0.01, None),
     (3, "LegendreHex", "LaxFriedrichs", (4, 2, 3), (1.0, 0.2, -0.4, 1.0, 3.0), None, None),
     (4, "LegendreHex", "Roe", (2, 3, 2), (1.0, 0.7, 0.1, 1.0, 1.0), 0.005, None),
     (2, "LegendreHex", "Roe", (6, 6, 6), (1.0, -0.3, 0.6, 1.0, 4.0), None, None),
