@@ -155,18 +155,22 @@ def batch_bench(args):
 
     dof = sum(n ** 3 * (p + 1) ** 3 for n, p, *_ in cases)
     K, W = args.steps, args.warmup
-    def batch_time(solvers):
+    tunings = {}
+
+    def batch_time(solvers, key):
+        # the warm-up call (4 steps at least) tunes the team sizes
         with dg_amd.Batch(solvers) as b:
-            b.step(W)
+            b.step(max(W, 4))
             t0 = time.perf_counter()
             b.step(K)
+            tunings[key] = b.tuning()
             return (time.perf_counter() - t0) / K
 
     s64 = make(None)
-    t64 = batch_time(s64)
+    t64 = batch_time(s64, "fp64")
     e64 = np.array([s.get_error()[0] for s in s64])
     s32 = make("FP32")
-    t32 = batch_time(s32)
+    t32 = batch_time(s32, "fp32")
     e32 = np.array([s.get_error()[0] for s in s32])
     d32 = np.array([float(np.max(np.abs(a.get_state() - b.get_state())) / np.max(np.abs(b.get_state())))
                     for a, b in zip(s32, s64)])
@@ -175,7 +179,7 @@ def batch_bench(args):
     # the same FP64 cases one after the other (dg_step per case), same steps
     seq = make(None)
     for s in seq:
-        s.take_time_step(0.0, W)
+        s.take_time_step(0.0, max(W, 4))
     hip.synchronize()
     t0 = time.perf_counter()
     for s in seq:
@@ -189,12 +193,14 @@ def batch_bench(args):
     ratio = e32 / e64
     out = {
         "metric": "aggregate DOF-updates/s per RK stage, batched uneven periodic vortex cases",
-        "value": dof / (t64 / 4), "unit": "DOF-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "value": dof / (t64 / 4), "unit": "DOF-updates/s", "n_gpus": 1, "steps": K, "warmup": max(W, 4),
         "ms_per_step": t64 * 1e3, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"batch_{n_cases}_vortex_cases_n8-32_p1-4_legendre_roe_rk4_cfl0.1-0.04",
                    "seed": args.seed, "cases": n_cases, "elements": int(sum(n ** 3 for n, *_ in cases)),
                    "dof": int(dof), "orders": {str(q): sum(1 for c in cases if c[1] == q) for q in range(1, 5)}},
         "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t64,
+        "tuning": {k: [{"order": o, "team": t, "ms_default_team": a, "ms_alternate_team": b2}
+                       for o, t, a, b2 in v] for k, v in tunings.items()},
         "max_rel_diff_vs_sequential": same,
         "fp32_form": {"value": dof / (t32 / 4), "ms_per_step": t32 * 1e3, "speedup_vs_fp64": t64 / t32,
                       "rho_l2_error_ratio_vs_fp64": {"median": float(np.median(ratio)), "max": float(np.max(ratio)),

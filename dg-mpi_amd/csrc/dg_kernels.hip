@@ -1436,10 +1436,10 @@ __device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int l
 // contractions) in single precision on FP32 copies of the state and the
 // tables; the residual is widened to FP64 before M^-1 and the RK update,
 // which read and write the FP64 registers.  Periodic box only.
-template <int P, int FLUX, bool MAPPED, class R>
+template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>()>
 __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T, const KGeom& G, unsigned e) {
     static_assert(sizeof(R) == 8 || !MAPPED, "the FP32 form is for the periodic box");
-    constexpr int TS = stage_team<P>();
+    static_assert(TS % WAVE == 0, "whole waves");
     using Dm = Dims<P>;
     using L = StageLDS<P, MAPPED>;
     using PF = StageLoads<P, TS>;
@@ -1555,8 +1555,8 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, con
 // tables and RK registers (BatchCase), so every element sees the stage
 // arithmetic of its case's own launch.  Positions past the last element
 // (the padding to a multiple of 8) return at once.
-template <int P, int FLUX, class R = double>
-__global__ __launch_bounds__(stage_team<P>()) void stage_batch_kernel(const BatchCase* __restrict__ cases,
+template <int P, int FLUX, class R = double, int TS = stage_team<P>()>
+__global__ __launch_bounds__(TS) void stage_batch_kernel(const BatchCase* __restrict__ cases,
                                                                       const unsigned* __restrict__ blk_case,
                                                                       unsigned nreal, int stage,
                                                                       const KTables* __restrict__ Tp,
@@ -1579,7 +1579,7 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_batch_kernel(const Batc
     A.alpha = stage == 2 ? 1.0 : 0.5;
     A.mode = stage == 0 ? MODE_RK_FIRST : (stage == 3 ? MODE_RK_LAST : MODE_RK_MID);
     const KGeom G = C->g;
-    stage_body<P, FLUX, false, R>(A, *Tp, G, brick_element(G, q - C->first_block));
+    stage_body<P, FLUX, false, R, TS>(A, *Tp, G, brick_element(G, q - C->first_block));
 }
 
 // dU = dt M^-1 R
@@ -1966,24 +1966,44 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
     return hipGetLastError();
 }
 
-template <int P>
-hipError_t stage_batch_p(int flux, int fp32, const BatchCase* cases, const unsigned* blk_case, unsigned nblocks,
-                         int stage, const KTables* t, hipStream_t st) {
-    const dim3 grid((nblocks + 7u) & ~7u), block(stage_team<P>());
+template <int P, int TS>
+hipError_t stage_batch_pt(int flux, int fp32, const BatchCase* cases, const unsigned* blk_case, unsigned nblocks,
+                          int stage, const KTables* t, hipStream_t st) {
+    const dim3 grid((nblocks + 7u) & ~7u), block(TS);
     if (fp32) {
         switch (flux) {
-        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, float>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, float>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, float>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
         }
     } else {
         switch (flux) {
-        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
         }
     }
     return hipGetLastError();
+}
+
+// the second team size the batched launch can run each order with (the
+// first is stage_team<P>); the batch picks per group by timing both
+template <int P>
+constexpr int alt_team() {
+    return P == 1 ? 128 : (P == 2 ? 64 : (P <= 4 ? 128 : 256));
+}
+
+template <int P>
+hipError_t stage_batch_p(int flux, int fp32, int variant, const BatchCase* cases, const unsigned* blk_case,
+                         unsigned nblocks, int stage, const KTables* t, hipStream_t st) {
+    if (variant == 1)
+        return stage_batch_pt<P, alt_team<P>()>(flux, fp32, cases, blk_case, nblocks, stage, t, st);
+    return stage_batch_pt<P, stage_team<P>()>(flux, fp32, cases, blk_case, nblocks, stage, t, st);
+}
+
+template <int P>
+constexpr int team_of(int variant) {
+    return variant == 1 ? alt_team<P>() : stage_team<P>();
 }
 
 }  // namespace
@@ -2007,11 +2027,22 @@ hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, c
     return hipSuccess;
 }
 
-hipError_t launch_stage_batch(int P, int flux, int fp32, const BatchCase* cases, const unsigned* blk_case,
-                              unsigned nblocks, int stage, const KTables* t, hipStream_t st) {
+hipError_t launch_stage_batch(int P, int flux, int fp32, int variant, const BatchCase* cases,
+                              const unsigned* blk_case, unsigned nblocks, int stage, const KTables* t, hipStream_t st) {
     if (nblocks == 0) return hipSuccess;
-    DG_DISPATCH_P(P, return stage_batch_p<PP>(flux, fp32, cases, blk_case, nblocks, stage, t, st));
+    DG_DISPATCH_P(P, return stage_batch_p<PP>(flux, fp32, variant, cases, blk_case, nblocks, stage, t, st));
     return hipSuccess;
+}
+
+int batch_team(int P, int variant) {
+    switch (P) {
+    case 1: return team_of<1>(variant);
+    case 2: return team_of<2>(variant);
+    case 3: return team_of<3>(variant);
+    case 4: return team_of<4>(variant);
+    case 5: return team_of<5>(variant);
+    default: return 0;
+    }
 }
 
 hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,

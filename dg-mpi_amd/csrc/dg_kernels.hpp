@@ -105,9 +105,12 @@ hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, c
                         hipStream_t st);
 // Batched stage over the cases of one (order, basis, flux, precision)
 // group (tables t shared): nblocks = total elements of the group,
-// blk_case[q] = case of traversal position q, stage 0..3 of RK4.
-hipError_t launch_stage_batch(int P, int flux, int fp32, const BatchCase* cases, const unsigned* blk_case,
-                              unsigned nblocks, int stage, const KTables* t, hipStream_t st);
+// blk_case[q] = case of traversal position q, stage 0..3 of RK4; variant
+// 0 or 1 picks one of two team sizes per order (batch_team).
+hipError_t launch_stage_batch(int P, int flux, int fp32, int variant, const BatchCase* cases,
+                              const unsigned* blk_case, unsigned nblocks, int stage, const KTables* t, hipStream_t st);
+// threads per element of the batched launch's variant (0 or 1) at order P
+int batch_team(int P, int variant);
 hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,
                            const KGeom& g, hipStream_t st);
 hipError_t launch_init_state(int P, double time, double* U, const KTables* t, const KGeom& g,

@@ -590,6 +590,8 @@ struct dg_batch {
         std::vector<int> members;             // indices into cases
         dg::BatchCase* dcases = nullptr;      // device descriptors
         unsigned* blk_case = nullptr;         // device: case (within the group) of each block
+        int variant = 0;                      // team-size variant of the launch (dg::batch_team)
+        double ms[2] = {0.0, 0.0};            // measured time of one RK4 step per variant
     };
     std::vector<dg_solver*> cases;
     std::vector<Group> groups;
@@ -598,6 +600,7 @@ struct dg_batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;               // one RK4 step of every group
     hipGraphExec_t exec = nullptr;
+    bool tuned = false;                       // variants chosen by timing
     std::string err;
 };
 
@@ -628,6 +631,70 @@ void upload_group(dg_batch* b, dg_batch::Group& g) {
     }
     hip_check(hipMemcpy(g.dcases, h.data(), h.size() * sizeof(BatchCase), hipMemcpyHostToDevice),
               "hipMemcpy(batch cases)");
+}
+
+// one RK4 step of every group, launched on the batch stream (captured or
+// eager)
+hipError_t launch_step(dg_batch* b) {
+    for (int st = 0; st < 4; ++st)
+        for (auto& g : b->groups) {
+            const hipError_t e = dg::launch_stage_batch(g.P, g.flux, g.fp32, g.variant, g.dcases, g.blk_case,
+                                                        g.nblocks, st, b->cases[g.members[0]]->dtab, b->stream);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+void capture_step(dg_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    const hipError_t le = launch_step(b);
+    const hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
+    hip_check(le, "batched stage launch (capture)");
+    hip_check(ce, "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+// Autotuning: four eager RK4 steps with the variants 0, 1, 0, 1 for every
+// group, each group's launches bracketed by events; the second measurement
+// of each variant counts (the first pays for code loading).  Each output
+// value is computed by one thread with the same operations whatever the
+// team size (each variant being compiled on its own, multiply-add fusion
+// may differ: round-off), so these are ordinary steps of the cases.
+void tune(dg_batch* b) {
+    const size_t ng = b->groups.size();
+    std::vector<hipEvent_t> ev(ng * 8);
+    for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    for (int k = 0; k < 4; ++k) {
+        const int v = k & 1;
+        for (auto& g : b->groups) g.variant = v;
+        for (int st = 0; st < 4; ++st)
+            for (size_t gi = 0; gi < ng; ++gi) {
+                auto& g = b->groups[gi];
+                hip_check(hipEventRecord(ev[gi * 8 + st * 2], b->stream), "hipEventRecord");
+                hip_check(dg::launch_stage_batch(g.P, g.flux, g.fp32, v, g.dcases, g.blk_case, g.nblocks, st,
+                                                 b->cases[g.members[0]]->dtab, b->stream),
+                          "batched stage launch");
+                hip_check(hipEventRecord(ev[gi * 8 + st * 2 + 1], b->stream), "hipEventRecord");
+            }
+        hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        for (size_t gi = 0; gi < ng; ++gi) {
+            double ms = 0.0;
+            for (int st = 0; st < 4; ++st) {
+                float t = 0.f;
+                hip_check(hipEventElapsedTime(&t, ev[gi * 8 + st * 2], ev[gi * 8 + st * 2 + 1]),
+                          "hipEventElapsedTime");
+                ms += t;
+            }
+            b->groups[gi].ms[v] = ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& g : b->groups) g.variant = g.ms[1] < g.ms[0] ? 1 : 0;
+    b->tuned = true;
 }
 
 void free_batch(dg_batch* b) {
@@ -706,19 +773,9 @@ int dg_batch_create(int num_cases, dg_solver* const* cases, dg_batch** out) {
         }
         b->dts.resize(b->cases.size());
         for (size_t i = 0; i < b->cases.size(); ++i) b->dts[i] = b->cases[i]->dt;
-        // one RK4 step of every group, captured once and replayed per step
-        hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        hipError_t le = hipSuccess;
-        for (int st = 0; st < 4 && le == hipSuccess; ++st)
-            for (auto& g : b->groups) {
-                le = dg::launch_stage_batch(g.P, g.flux, g.fp32, g.dcases, g.blk_case, g.nblocks, st,
-                                            b->cases[g.members[0]]->dtab, b->stream);
-                if (le != hipSuccess) break;
-            }
-        const hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
-        hip_check(le, "batched stage launch (capture)");
-        hip_check(ce, "hipStreamEndCapture");
-        hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        // one RK4 step of every group (variant 0 until tuned), captured once
+        // and replayed per step
+        capture_step(b);
     });
     if (rc != DG_OK) {
         free_batch(b);
@@ -748,7 +805,15 @@ int dg_batch_step(dg_batch* b, int nsteps) {
             for (auto& g : b->groups) upload_group(b, g);
             for (size_t i = 0; i < b->cases.size(); ++i) b->dts[i] = b->cases[i]->dt;
         }
-        for (int n = 0; n < nsteps; ++n) hip_check(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+        int n0 = 0;
+        if (!b->tuned && nsteps >= 4) {
+            // the first four steps of a long enough call pick each group's
+            // team size, then the step is captured again with the choices
+            tune(b);
+            capture_step(b);
+            n0 = 4;
+        }
+        for (int n = n0; n < nsteps; ++n) hip_check(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
         hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         for (dg_solver* s : b->cases)
             for (int n = 0; n < nsteps; ++n) {   // as step_once advances it
@@ -766,5 +831,21 @@ int dg_batch_destroy(dg_batch* b) {
 }
 
 const char* dg_batch_last_error(const dg_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
+
+int dg_batch_tuning(const dg_batch* b, int* num_groups, int* order, int* team, double* ms_default,
+                    double* ms_alternate) {
+    return guarded_to(b ? const_cast<dg_batch*>(b)->err : g_batch_error, [&] {
+        need(b, "batch");
+        need(num_groups, "num_groups");
+        *num_groups = (int)b->groups.size();
+        for (size_t i = 0; i < b->groups.size(); ++i) {
+            const auto& g = b->groups[i];
+            if (order) order[i] = g.P;
+            if (team) team[i] = dg::batch_team(g.P, g.variant);
+            if (ms_default) ms_default[i] = b->tuned ? g.ms[0] : 0.0;
+            if (ms_alternate) ms_alternate[i] = b->tuned ? g.ms[1] : 0.0;
+        }
+    });
+}
 
 }  // extern "C"

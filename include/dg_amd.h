@@ -201,6 +201,15 @@ int dg_batch_destroy(dg_batch* b);
 /* Last error of the batch (NULL: of the last failed dg_batch_create on this
  * thread).  Never returns NULL. */
 const char* dg_batch_last_error(const dg_batch* b);
+/* Each (order, basis, flux, precision) group's launch has two team sizes;
+ * the first dg_batch_step call with nsteps >= 4 runs its first four steps
+ * with the two alternately, keeps the faster per group (the second timing
+ * of each) and captures the step again.  This reports, per group, the
+ * order, the team size in use and the two timings (ms per RK4 step of the
+ * group; 0 before tuning).  Arrays of *num_groups entries; NULL arrays:
+ * only the count. */
+int dg_batch_tuning(const dg_batch* b, int* num_groups, int* order, int* team, double* ms_default,
+                    double* ms_alternate);
 
 /* Learned closure for coarse runs (no reference counterpart).  A per-element
  * MLP -- 70 features -> 128 -> 128 -> 40, GELU, BF16 weights with FP32

@@ -96,3 +96,40 @@ def test_batch_input_checks(gpu):
             b.step(1)
     for s in (a, warped, fe, nodt):
         s.close()
+
+
+def test_batch_tuning_keeps_results(gpu):
+    """A call with nsteps >= 4 times the two team sizes of every group on its
+    first four steps and keeps the faster; the cases still end where their
+    own dg_step takes them, to round-off (a team size of its own compiles
+    to its own multiply-add fusion: ~1e-16 relative in FP64, ~1e-8 for the
+    FP32-form case)."""
+    cases = [CASES[0], CASES[2], CASES[3], CASES[5], CASES[6]]
+    batch_s = [dg_amd.Solver(toml(c)) for c in cases]
+    solo_s = [dg_amd.Solver(toml(c)) for c in cases]
+    for s in batch_s + solo_s:
+        s.init_state_from_fcn()
+    with dg_amd.Batch(batch_s) as b:
+        assert all(t[2] == 0.0 for t in b.tuning())
+        b.step(6)
+        tun = b.tuning()
+        print("tuning (order, team, ms default, ms alternate):", tun, flush=True)
+        assert len(tun) == len(cases)
+        for order, team, t0, t1 in tun:
+            assert t0 > 0.0 and t1 > 0.0
+            assert team in (64, 128, 256, 512)
+        for s in solo_s:
+            s.take_time_step(0.0, 6)
+        tol = [1e-14 if c[6] is None else 2e-6 for c in cases]
+        for sb, so, t in zip(batch_s, solo_s, tol):
+            assert sb.time == so.time
+            Ub, Uo = sb.get_state(), so.get_state()
+            assert np.max(np.abs(Ub - Uo)) / np.max(np.abs(Uo)) < t
+        b.step(2)                               # replays of the re-captured step
+        for s in solo_s:
+            s.take_time_step(0.0, 2)
+        for sb, so, t in zip(batch_s, solo_s, tol):
+            Ub, Uo = sb.get_state(), so.get_state()
+            assert np.max(np.abs(Ub - Uo)) / np.max(np.abs(Uo)) < t
+    for s in batch_s + solo_s:
+        s.close()
