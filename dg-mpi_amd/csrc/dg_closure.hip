@@ -200,50 +200,51 @@ __global__ void to_bf16_kernel(const float* __restrict__ p, uint16_t* __restrict
 // ---------------------------------------------------------------------------
 
 // face mean of element block u (8 x 5) on face (d, side), variable s:
-// (1/4) sum_ab w_a w_b U(face point (a, b)), the (p+2) = 3-point rule
-__device__ double face_mean(const dg::KTables& T, const double* u, int d, int side, int s) {
+// (1/4) sum_ab w_a w_b U(face point (a, b)) on the (p+2) = 3-point rule,
+// factored as sum_b u_b phi_r(+-1) m_t1 m_t2 with the basis means
+// m_j = (1/2) sum_a w_a phi_j(x_a)
+__device__ double face_mean(const dg::KTables& T, const double* m, const double* u, int d, int side, int s) {
     const double* phi = side ? T.phip : T.phim;
     double acc = 0.;
-    for (int a = 0; a < 3; ++a)
-        for (int c = 0; c < 3; ++c) {
-            double v = 0.;
-            for (int k = 0; k < 2; ++k)
-                for (int j = 0; j < 2; ++j)
-                    for (int i = 0; i < 2; ++i) {
-                        const int b = i + 2 * j + 4 * k;
-                        // normal index and the two tangential ones
-                        const int r = d == 0 ? i : (d == 1 ? j : k);
-                        const int t1 = d == 0 ? j : i, t2 = d == 2 ? j : k;
-                        v += u[b * 5 + s] * phi[r] * T.B[a][t1] * T.B[c][t2];
-                    }
-            acc += T.w[a] * T.w[c] * v;
-        }
-    return 0.25 * acc;
+    for (int k = 0; k < 2; ++k)
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i) {
+                const int b = i + 2 * j + 4 * k;
+                const int r = d == 0 ? i : (d == 1 ? j : k);
+                const int t1 = d == 0 ? j : i, t2 = d == 2 ? j : k;
+                acc = fma(u[b * 5 + s], phi[r] * m[t1] * m[t2], acc);
+            }
+    return acc;
 }
 
 // X[e] = own 40 coefficients, then for the faces x-, x+, y-, y+, z-, z+ the
-// jump of the face-mean state (neighbour's matching face minus own), 5 each
+// jump of the face-mean state (neighbour's matching face minus own), 5
+// each; one thread per feature
 __global__ void features_kernel(const double* __restrict__ U, float* __restrict__ X,
                                 const dg::KTables* __restrict__ Tp, dg::KGeom G) {
-    const long e = (long)blockIdx.x * TB + threadIdx.x;
-    if (e >= (long)G.E) return;
-    const dg::KTables& T = *Tp;
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    if (t >= (long)G.E * NF) return;
+    const long e = t / NF;
+    const int c = (int)(t % NF);
     const double* u = U + e * 40;
-    float* x = X + e * NF;
-    for (int c = 0; c < 40; ++c) x[c] = (float)u[c];
+    if (c < 40) {
+        X[t] = (float)u[c];
+        return;
+    }
+    const dg::KTables& T = *Tp;
+    double m[2];
+    for (int jj = 0; jj < 2; ++jj) {
+        double a = 0.;
+        for (int q = 0; q < 3; ++q) a += T.w[q] * T.B[q][jj];
+        m[jj] = 0.5 * a;
+    }
+    const int f = (c - 40) / 5, s = (c - 40) % 5, d = f >> 1, side = f & 1;
     const int nx = G.ne[0], ny = G.ne[1], nz = G.ne[2];
-    const int ix = (int)(e % nx), iy = (int)((e / nx) % ny), iz = (int)(e / ((long)nx * ny));
-    for (int d = 0; d < 3; ++d)
-        for (int side = 0; side < 2; ++side) {
-            int c[3] = {ix, iy, iz};
-            const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
-            c[d] = side ? (c[d] + 1) % nd : (c[d] + nd - 1) % nd;
-            const long en = c[0] + (long)nx * (c[1] + (long)ny * c[2]);
-            const double* un = U + en * 40;
-            for (int s = 0; s < 5; ++s)
-                x[40 + (d * 2 + side) * 5 + s] =
-                    (float)(face_mean(T, un, d, 1 - side, s) - face_mean(T, u, d, side, s));
-        }
+    int cc[3] = {(int)(e % nx), (int)((e / nx) % ny), (int)(e / ((long)nx * ny))};
+    const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
+    cc[d] = side ? (cc[d] + 1) % nd : (cc[d] + nd - 1) % nd;
+    const long en = cc[0] + (long)nx * (cc[1] + (long)ny * cc[2]);
+    X[t] = (float)(face_mean(T, m, U + en * 40, d, 1 - side, s) - face_mean(T, m, u, d, side, s));
 }
 
 // coarse state = Legendre modes (i, j, k) <= 1 of the fine state (the L2
@@ -266,6 +267,100 @@ __global__ void target_kernel(const double* __restrict__ Df, const double* __res
     const int r = (int)(t % 40), b = r / 5, s = r % 5;
     const int i = b & 1, j = (b >> 1) & 1, k = b >> 2, n = Pf + 1;
     Y[t] = (float)(Df[(e * n * n * n + i + n * j + n * n * k) * 5 + s] - Dc[t]);
+}
+
+// Fused inference for the closure step: a block of 128 threads takes 32
+// rows through normalisation, the three layers (BF16 weights, FP32
+// accumulation, activations in LDS), the de-normalisation and the
+// fallback bound, and writes C.  Thread n owns output n of a layer for all
+// 32 rows (32 accumulators in registers); the weights stream from global
+// memory, coalesced along n.
+constexpr int IR = 32, IT = 128, KF = 72;   // KF: feature rows padded to whole float4s
+
+// acc[r] += sum_k A[r][k] W[k][n] over k = 0 .. K-1 (K a multiple of 4),
+// the activations read four at a time from LDS
+template <int R, int K, int LDA, int LDW>
+__device__ __forceinline__ void layer_acc(const float (*A)[LDA], const uint16_t* __restrict__ W, int n, int r0,
+                                          float* acc) {
+    for (int k = 0; k < K; k += 4) {
+        const float w0 = from_bf16(W[(k + 0) * LDW + n]), w1 = from_bf16(W[(k + 1) * LDW + n]);
+        const float w2 = from_bf16(W[(k + 2) * LDW + n]), w3 = from_bf16(W[(k + 3) * LDW + n]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float4 a = *reinterpret_cast<const float4*>(&A[r0 + r][k]);
+            acc[r] = fmaf(a.x, w0, acc[r]);
+            acc[r] = fmaf(a.y, w1, acc[r]);
+            acc[r] = fmaf(a.z, w2, acc[r]);
+            acc[r] = fmaf(a.w, w3, acc[r]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(IT) void mlp_infer_kernel(const float* __restrict__ X, const uint16_t* __restrict__ W,
+                                                       const float* __restrict__ P, const float* __restrict__ xs,
+                                                       float* __restrict__ C, double bound, long rows,
+                                                       unsigned* __restrict__ count) {
+    __shared__ alignas(16) float A0[IR][KF];
+    __shared__ alignas(16) float H[2][IR][NH];
+    __shared__ float Oo[IR][NO];
+    __shared__ alignas(16) uint16_t W1p[KF * NH];       // W1 with two zero rows
+    const int t = threadIdx.x;
+    const long r0 = (long)blockIdx.x * IR;
+    const int nr = (int)((rows - r0) < IR ? (rows - r0) : IR);
+    for (int i = t; i < IR * KF; i += IT) {
+        const int r = i / KF, k = i % KF;
+        A0[r][k] = (r < nr && k < NF) ? (X[(r0 + r) * NF + k] - xs[k]) * xs[NF + k] : 0.f;
+    }
+    for (int i = t; i < KF * NH; i += IT) W1p[i] = i < NF * NH ? W[OFF_W1 + i] : (uint16_t)0;
+    __syncthreads();
+    float acc[IR];
+    // layer 1: 70 -> 128, GELU
+    {
+        const float b = P[OFF_B1 + t];
+#pragma unroll
+        for (int r = 0; r < IR; ++r) acc[r] = b;
+        layer_acc<IR, KF, KF, NH>(A0, W1p, t, 0, acc);
+#pragma unroll
+        for (int r = 0; r < IR; ++r) H[0][r][t] = gelu(acc[r]);
+    }
+    __syncthreads();
+    // layer 2: 128 -> 128, GELU
+    {
+        const float b = P[OFF_B2 + t];
+#pragma unroll
+        for (int r = 0; r < IR; ++r) acc[r] = b;
+        layer_acc<IR, NH, NH, NH>(H[0], W + OFF_W2, t, 0, acc);
+#pragma unroll
+        for (int r = 0; r < IR; ++r) H[1][r][t] = gelu(acc[r]);
+    }
+    __syncthreads();
+    // layer 3: 128 -> 40, de-normalised; 3 x 40 threads, 11 / 11 / 10 rows
+    if (t < 3 * NO) {
+        const int n = t % NO, g = t / NO, ra = g * 11, rn = g < 2 ? 11 : IR - 22;
+        const float b = P[OFF_B3 + n];
+        float a3[11];
+#pragma unroll
+        for (int r = 0; r < 11; ++r) a3[r] = b;
+        if (rn == 11) layer_acc<11, NH, NH, NO>(H[1], W + OFF_W3, n, ra, a3);
+        else layer_acc<10, NH, NH, NO>(H[1], W + OFF_W3, n, ra, a3);
+        const float ys = xs[2 * NF + n];
+        for (int r = 0; r < rn; ++r) Oo[ra + r][n] = a3[r] * ys;
+    }
+    __syncthreads();
+    // fallback bound per row, then the store
+    if (t < IR) {
+        double n2 = 0.;
+        for (int i = 0; i < NO; ++i) n2 += (double)Oo[t][i] * Oo[t][i];
+        if (t < nr && !(sqrt(n2) <= bound)) {
+            for (int i = 0; i < NO; ++i) Oo[t][i] = 0.f;
+            atomicAdd(count, 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < IR * NO; i += IT) {
+        const int r = i / NO;
+        if (r < nr) C[(r0 + r) * NO + (i % NO)] = Oo[r][i % NO];
+    }
 }
 
 // C = O * yscale (the network's normalised outputs, de-normalised)
@@ -631,8 +726,8 @@ int dg_closure_features(dg_solver* s, const double* U_dev, float* X_dev) {
         need_ptr(U_dev, "U");
         need_ptr(X_dev, "X");
         hcheck(hipSetDevice(s->device), "hipSetDevice");
-        hipLaunchKernelGGL(features_kernel, dim3(nblk(s->geo.E)), dim3(TB), 0, s->stream, U_dev, X_dev, s->dtab,
-                           s->geo);
+        hipLaunchKernelGGL(features_kernel, dim3(nblk((long)s->geo.E * NF)), dim3(TB), 0, s->stream, U_dev, X_dev,
+                           s->dtab, s->geo);
         hcheck(hipGetLastError(), "features kernel");
         hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     });
@@ -714,14 +809,14 @@ int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int n
         hcheck(hipMalloc(&dU, (size_t)n * sizeof(double)), "hipMalloc");
         hcheck(hipMalloc(&cnt, sizeof(unsigned)), "hipMalloc");
         hcheck(hipMemset(cnt, 0, sizeof(unsigned)), "hipMemset");
-        ensure_acts(m, E);
+        // the network's parameters are written by its own stream (training)
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
         auto stage = [&](const double* Ust, double* Uout, int mode, double alpha) {
-            hipLaunchKernelGGL(features_kernel, dim3(nblk(E)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
-            hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
-            forward(m, X, E);
-            hipLaunchKernelGGL(descale_kernel, dim3(nblk(n)), dim3(TB), 0, m->stream, m->O, m->xs + 2 * NF, C, n);
-            hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
-            hipLaunchKernelGGL(bound_kernel, dim3(nblk(E)), dim3(TB), 0, s->stream, C, bound, E, cnt);
+            // everything on the solver's stream: features, the fused network
+            // with its fallback, the residual, M^-1 and the update
+            hipLaunchKernelGGL(features_kernel, dim3(nblk(E * NF)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
+            hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)((E + IR - 1) / IR)), dim3(IT), 0, s->stream, X,
+                               m->pb, m->p, m->xs, C, bound, E, cnt);
             dg::StageArgs a{};
             a.Ust = Ust;
             a.R = s->Ub;
