@@ -399,7 +399,9 @@ def main():
             "dtype": "f32 traces/fluxes/contractions, f64 RK registers, M^-1 and reductions",
             "kernel": f"stage_kernel<{order},Roe,fp32>",
             "l2_error_vs_exact": [float(x) for x in err32],
-            "l2_error_ratio_vs_fp64": [float(a / b) if b > 0 else None for a, b in zip(err32, err)],
+            # the extruded vortex has w = 0 exactly: FP64 keeps it at round-off (~1e-16) and
+            # FP32 at ~1e-8, so the ratio is reported for the variables with a real error
+            "l2_error_ratio_vs_fp64": [float(a / b) if b > 1e-12 else None for a, b in zip(err32, err)],
             "state_max_rel_diff_vs_fp64": diff,
         }
         s32.close()
