@@ -269,105 +269,135 @@ __global__ void target_kernel(const double* __restrict__ Df, const double* __res
     Y[t] = (float)(Df[(e * n * n * n + i + n * j + n * n * k) * 5 + s] - Dc[t]);
 }
 
-// Fused inference for the closure step: a block of 128 threads takes 32
-// rows through normalisation, the three layers (BF16 weights, FP32
-// accumulation, activations in LDS), the de-normalisation and the
-// fallback bound, and writes C.  Thread n owns output n of a layer for all
-// 32 rows (32 accumulators in registers); the weights stream from global
-// memory, coalesced along n.
-constexpr int IR = 32, IT = 128, KF = 72;   // KF: feature rows padded to whole float4s
+// Fused inference for the closure step and dg_mlp_forward, on the matrix
+// cores: v_mfma_f32_16x16x16_bf16 tiles (bf16 operands, FP32 accumulation)
+// for the three layers, with the normalisation, GELU, de-normalisation and
+// the fallback bound around them.  The grid is persistent (at most one
+// block per CU): a block stages the three weight matrices once, transposed
+// to [output][input] in LDS, then walks 64-row tiles; wave w owns rows
+// 16w .. 16w+15 of a tile through all three layers.  Activations enter each
+// layer rounded to bf16.
+//
+// Operand layout of the 16x16x16 bf16 MFMA (lane l, element q = 0..3):
+// A[l % 16][4 (l / 16) + q], B[4 (l / 16) + q][l % 16]; the result lane l
+// holds D[4 (l / 16) + q][l % 16].
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int MT = 256, MR = 64;               // threads, rows per tile
+constexpr int K1 = 80;                         // layer-1 inputs padded to whole 16-wide k steps
+constexpr int L1S = K1 + 4, L2S = NH + 4;      // LDS row strides (bf16), padded off the bank period
+constexpr int N3 = 48;                         // layer-3 outputs padded to whole 16-column tiles
+constexpr int MAX_INFER_BLOCKS = 256;
 
-// acc[r] += sum_k A[r][k] W[k][n] over k = 0 .. K-1 (K a multiple of 4),
-// the activations read four at a time from LDS
-template <int R, int K, int LDA, int LDW>
-__device__ __forceinline__ void layer_acc(const float (*A)[LDA], const uint16_t* __restrict__ W, int n, int r0,
-                                          float* acc) {
-    for (int k = 0; k < K; k += 4) {
-        const float w0 = from_bf16(W[(k + 0) * LDW + n]), w1 = from_bf16(W[(k + 1) * LDW + n]);
-        const float w2 = from_bf16(W[(k + 2) * LDW + n]), w3 = from_bf16(W[(k + 3) * LDW + n]);
+struct InferLDS {
+    uint16_t w1[NH][L1S];                      // W1^T; inputs k >= NF are zero
+    uint16_t w2[NH][L2S];                      // W2^T
+    uint16_t w3[N3][L2S];                      // W3^T; outputs n >= NO are zero
+    uint16_t a0[MR][L1S];                      // normalised features
+    uint16_t h1[MR][L2S];
+    uint16_t h2[MR][L2S];
+    float o[MR][NO + 1];                       // de-normalised outputs
+};
+
+__device__ inline bf16x4 ld_bf16x4(const uint16_t* q) { return *reinterpret_cast<const bf16x4*>(q); }
+
+// acc[j] += A[16 rows][K] x W^T[16 j .. 16 j + 15][K] for j < NT
+template <int NT, int K, int LA, int LW>
+__device__ __forceinline__ void mfma_layer(const uint16_t (*A)[LA], const uint16_t (*Wt)[LW], int rw, int lane,
+                                           f32x4* acc) {
+    const int c = lane & 15, kq = 4 * (lane >> 4);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float4 a = *reinterpret_cast<const float4*>(&A[r0 + r][k]);
-            acc[r] = fmaf(a.x, w0, acc[r]);
-            acc[r] = fmaf(a.y, w1, acc[r]);
-            acc[r] = fmaf(a.z, w2, acc[r]);
-            acc[r] = fmaf(a.w, w3, acc[r]);
-        }
+    for (int kk = 0; kk < K; kk += 16) {
+        const bf16x4 a = ld_bf16x4(&A[rw + c][kk + kq]);
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ld_bf16x4(&Wt[16 * j + c][kk + kq]), acc[j], 0, 0,
+                                                              0);
     }
 }
 
-__global__ __launch_bounds__(IT) void mlp_infer_kernel(const float* __restrict__ X, const uint16_t* __restrict__ W,
+__global__ __launch_bounds__(MT) void mlp_infer_kernel(const float* __restrict__ X, const uint16_t* __restrict__ W,
                                                        const float* __restrict__ P, const float* __restrict__ xs,
                                                        float* __restrict__ C, double bound, long rows,
                                                        unsigned* __restrict__ count) {
-    __shared__ alignas(16) float A0[IR][KF];
-    __shared__ alignas(16) float H[2][IR][NH];
-    __shared__ float Oo[IR][NO];
-    __shared__ alignas(16) uint16_t W1p[KF * NH];       // W1 with two zero rows
-    const int t = threadIdx.x;
-    const long r0 = (long)blockIdx.x * IR;
-    const int nr = (int)((rows - r0) < IR ? (rows - r0) : IR);
-    for (int i = t; i < IR * KF; i += IT) {
-        const int r = i / KF, k = i % KF;
-        A0[r][k] = (r < nr && k < NF) ? (X[(r0 + r) * NF + k] - xs[k]) * xs[NF + k] : 0.f;
+    __shared__ alignas(16) InferLDS L;
+    const int t = threadIdx.x, lane = t & 63, rw = 16 * (t >> 6);
+    const int c = lane & 15, rq = 4 * (lane >> 4);
+    // weights, transposed; the global reads run along the output index
+    for (int i = t; i < K1 * NH; i += MT) {
+        const int k = i / NH, n = i % NH;
+        L.w1[n][k] = k < NF ? W[OFF_W1 + k * NH + n] : (uint16_t)0;
     }
-    for (int i = t; i < KF * NH; i += IT) W1p[i] = i < NF * NH ? W[OFF_W1 + i] : (uint16_t)0;
-    __syncthreads();
-    float acc[IR];
-    // layer 1: 70 -> 128, GELU
-    {
-        const float b = P[OFF_B1 + t];
-#pragma unroll
-        for (int r = 0; r < IR; ++r) acc[r] = b;
-        layer_acc<IR, KF, KF, NH>(A0, W1p, t, 0, acc);
-#pragma unroll
-        for (int r = 0; r < IR; ++r) H[0][r][t] = gelu(acc[r]);
+    for (int i = t; i < NH * NH; i += MT) L.w2[i % NH][i / NH] = W[OFF_W2 + i];
+    for (int i = t; i < NH * N3; i += MT) {
+        const int k = i / N3, n = i % N3;
+        L.w3[n][k] = n < NO ? W[OFF_W3 + k * NO + n] : (uint16_t)0;
     }
-    __syncthreads();
-    // layer 2: 128 -> 128, GELU
-    {
-        const float b = P[OFF_B2 + t];
-#pragma unroll
-        for (int r = 0; r < IR; ++r) acc[r] = b;
-        layer_acc<IR, NH, NH, NH>(H[0], W + OFF_W2, t, 0, acc);
-#pragma unroll
-        for (int r = 0; r < IR; ++r) H[1][r][t] = gelu(acc[r]);
-    }
-    __syncthreads();
-    // layer 3: 128 -> 40, de-normalised; 3 x 40 threads, 11 / 11 / 10 rows
-    if (t < 3 * NO) {
-        const int n = t % NO, g = t / NO, ra = g * 11, rn = g < 2 ? 11 : IR - 22;
-        const float b = P[OFF_B3 + n];
-        float a3[11];
-#pragma unroll
-        for (int r = 0; r < 11; ++r) a3[r] = b;
-        if (rn == 11) layer_acc<11, NH, NH, NO>(H[1], W + OFF_W3, n, ra, a3);
-        else layer_acc<10, NH, NH, NO>(H[1], W + OFF_W3, n, ra, a3);
-        const float ys = xs[2 * NF + n];
-        for (int r = 0; r < rn; ++r) Oo[ra + r][n] = a3[r] * ys;
-    }
-    __syncthreads();
-    // fallback bound per row, then the store
-    if (t < IR) {
-        double n2 = 0.;
-        for (int i = 0; i < NO; ++i) n2 += (double)Oo[t][i] * Oo[t][i];
-        if (t < nr && !(sqrt(n2) <= bound)) {
-            for (int i = 0; i < NO; ++i) Oo[t][i] = 0.f;
-            atomicAdd(count, 1u);
+    const long ntile = (rows + MR - 1) / MR;
+    for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const long r0 = tile * MR;
+        const int nr = (int)(rows - r0 < MR ? rows - r0 : MR);
+        __syncthreads();                       // the staged weights; the previous tile's a0 and o are read
+        for (int i = t; i < MR * K1; i += MT) {
+            const int r = i / K1, k = i % K1;
+            L.a0[r][k] = (r < nr && k < NF) ? to_bf16((X[(r0 + r) * NF + k] - xs[k]) * xs[NF + k]) : (uint16_t)0;
         }
+        __syncthreads();
+        f32x4 acc[8];
+        // layer 1: NF -> NH, GELU
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float b = P[OFF_B1 + 16 * j + c];
+            acc[j] = f32x4{b, b, b, b};
+        }
+        mfma_layer<8, K1, L1S, L1S>(L.a0, L.w1, rw, lane, acc);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) L.h1[rw + rq + q][16 * j + c] = to_bf16(gelu(acc[j][q]));
+        __syncthreads();
+        // layer 2: NH -> NH, GELU
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float b = P[OFF_B2 + 16 * j + c];
+            acc[j] = f32x4{b, b, b, b};
+        }
+        mfma_layer<8, NH, L2S, L2S>(L.h1, L.w2, rw, lane, acc);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) L.h2[rw + rq + q][16 * j + c] = to_bf16(gelu(acc[j][q]));
+        __syncthreads();
+        // layer 3: NH -> NO, de-normalised
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int n = 16 * j + c;
+            const float b = n < NO ? P[OFF_B3 + n] : 0.f;
+            acc[j] = f32x4{b, b, b, b};
+        }
+        mfma_layer<3, NH, L2S, L2S>(L.h2, L.w3, rw, lane, acc);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int n = 16 * j + c;
+            if (n < NO) {
+                const float ys = xs[2 * NF + n];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) L.o[rw + rq + q][n] = acc[j][q] * ys;
+            }
+        }
+        __syncthreads();
+        // fallback bound per row (count null: no bound), then the store
+        if (count && t < MR) {
+            double n2 = 0.;
+            for (int i = 0; i < NO; ++i) n2 += (double)L.o[t][i] * L.o[t][i];
+            if (t < nr && !(sqrt(n2) <= bound)) {
+                for (int i = 0; i < NO; ++i) L.o[t][i] = 0.f;
+                atomicAdd(count, 1u);
+            }
+        }
+        __syncthreads();
+        for (int i = t; i < nr * NO; i += MT) C[r0 * NO + i] = L.o[i / NO][i % NO];
     }
-    __syncthreads();
-    for (int i = t; i < IR * NO; i += IT) {
-        const int r = i / NO;
-        if (r < nr) C[(r0 + r) * NO + (i % NO)] = Oo[r][i % NO];
-    }
-}
-
-// C = O * yscale (the network's normalised outputs, de-normalised)
-__global__ void descale_kernel(const float* __restrict__ O, const float* __restrict__ ys, float* __restrict__ C,
-                               long n) {
-    const long t = (long)blockIdx.x * TB + threadIdx.x;
-    if (t < n) C[t] = O[t] * ys[t % NO];
 }
 
 // Y = trunc(Df) - Dc + (trunc(Uf) - Uc) / tau, for a coarse state Uc of
@@ -450,6 +480,15 @@ void free_acts(dg_mlp* m) {
     m->Z1 = m->H1 = m->Z2 = m->H2 = m->O = m->G3 = m->G2 = m->G1 = m->Xb = m->Yb = nullptr;
     m->part = nullptr;
     m->cap = 0;
+}
+
+// the inference kernel on a persistent grid of at most one block per CU
+void launch_infer(const dg_mlp* m, const float* X, float* C, double bound, long rows, unsigned* count,
+                  hipStream_t st) {
+    const long ntile = (rows + MR - 1) / MR;
+    const unsigned grid = (unsigned)(ntile < MAX_INFER_BLOCKS ? ntile : MAX_INFER_BLOCKS);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(MT), 0, st, X, m->pb, m->p, m->xs, C, bound, rows, count);
+    hcheck(hipGetLastError(), "inference kernel");
 }
 
 void ensure_acts(dg_mlp* m, long rows) {
@@ -665,11 +704,8 @@ int dg_mlp_forward(dg_mlp* m, const float* X_dev, float* Y_dev, long rows) {
         need_ptr(Y_dev, "Y");
         if (rows < 1) throw std::invalid_argument("rows < 1");
         hcheck(hipSetDevice(m->device), "hipSetDevice");
-        ensure_acts(m, rows);
-        forward(m, X_dev, rows);
-        hipLaunchKernelGGL(descale_kernel, dim3(nblk(rows * NO)), dim3(TB), 0, m->stream, m->O, m->xs + 2 * NF, Y_dev,
-                           rows * NO);
-        hcheck(hipGetLastError(), "descale kernel");
+        // the closure step's inference kernel, without the fallback bound
+        launch_infer(m, X_dev, Y_dev, 0.0, rows, nullptr, m->stream);
         hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
     });
 }
@@ -815,8 +851,7 @@ int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int n
             // everything on the solver's stream: features, the fused network
             // with its fallback, the residual, M^-1 and the update
             hipLaunchKernelGGL(features_kernel, dim3(nblk(E * NF)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
-            hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)((E + IR - 1) / IR)), dim3(IT), 0, s->stream, X,
-                               m->pb, m->p, m->xs, C, bound, E, cnt);
+            launch_infer(m, X, C, bound, E, cnt, s->stream);
             dg::StageArgs a{};
             a.Ust = Ust;
             a.R = s->Ub;

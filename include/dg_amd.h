@@ -233,7 +233,8 @@ int dg_mlp_num_params(long* n);
 int dg_mlp_get_params(const dg_mlp* m, float* host);
 int dg_mlp_set_params(dg_mlp* m, const float* host);
 int dg_mlp_set_scaling(dg_mlp* m, const float* x_mean, const float* x_scale, const float* y_scale);
-/* Y_dev = C(X_dev), de-normalised */
+/* Y_dev = C(X_dev), de-normalised: the closure step's inference kernel
+ * (matrix cores: bf16 weights and layer inputs, FP32 accumulation) */
 int dg_mlp_forward(dg_mlp* m, const float* X_dev, float* Y_dev, long rows);
 /* gradient (host, num_params floats) and loss of the mean squared error
  * in normalised target units over all rows */

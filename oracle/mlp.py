@@ -5,9 +5,13 @@ the checker of the GPU kernels, never the product path).
 (round to nearest even) for the forward pass and for the backward pass to
 the inputs, FP32 master weights for the weight gradients; inputs normalised
 as (x - mean) * scale, outputs in units of the target scale; loss = mean
-squared error over rows and outputs.  Computed in float64 here, so the GPU's
-FP32 accumulation differs by rounding only.  Pinned by finite differences of
-the loss (tests/test_oracle_mlp.py)."""
+squared error over rows and outputs.  round_acts=True also rounds each
+layer's input (normalised features, hidden activations) to bfloat16, as the
+inference kernel (dg_mlp_forward and the closure step) feeds the matrix
+cores; training keeps FP32 activations.  Computed in float64 here, so the
+GPU's FP32 accumulation differs by rounding only.  Pinned by finite
+differences of the loss and by the size of the activation rounding
+(tests/test_oracle_mlp.py)."""
 import numpy as np
 from scipy.special import erf
 
@@ -40,15 +44,16 @@ def gelu_d(z):
     return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
 
 
-def forward(p, X, x_mean, x_scale, round_weights=True):
+def forward(p, X, x_mean, x_scale, round_weights=True, round_acts=False):
     P = unpack(p)
     r = bf16 if round_weights else (lambda w: np.asarray(w, dtype=np.float64))
+    ra = bf16 if round_acts else (lambda a: a)
     A0 = (np.asarray(X, np.float64) - x_mean) * x_scale
-    Z1 = A0 @ r(P["W1"]) + P["b1"]
+    Z1 = ra(A0) @ r(P["W1"]) + P["b1"]
     H1 = gelu(Z1)
-    Z2 = H1 @ r(P["W2"]) + P["b2"]
+    Z2 = ra(H1) @ r(P["W2"]) + P["b2"]
     H2 = gelu(Z2)
-    O = H2 @ r(P["W3"]) + P["b3"]
+    O = ra(H2) @ r(P["W3"]) + P["b3"]
     return O, (A0, Z1, H1, Z2, H2)
 
 

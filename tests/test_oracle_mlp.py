@@ -56,3 +56,50 @@ def test_gelu_derivative():
     h = 1e-6
     fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
     assert np.max(np.abs(fd - M.gelu_d(z))) < 1e-8
+
+
+def test_activation_rounding_stays_within_its_propagated_bound():
+    """forward(round_acts=True) rounds the input of each layer to bfloat16
+    (8 significant bits: relative error at most 2^-8 per value).
+    First-order propagation through |W| and max |GELU'| < 1.13 bounds the
+    output change; the change must stay under that bound; its RMS must
+    match the random-rounding estimate (independent errors, uniform within
+    half a bf16 ulp: variance ulp(a)^2 / 12, propagated through W^2 and
+    GELU'(z)^2) within 30 %, which fails if a layer's rounding is skipped or
+    doubled; and with activations that are already bfloat16 numbers (bf16
+    inputs, identity scaling, the hidden weights zero) nothing changes."""
+    p, X, _, xm, xs, _ = data(rows=64, seed=5)
+    O, (A0, Z1, H1, Z2, H2) = M.forward(p, X, xm, xs)
+    Or, _ = M.forward(p, X, xm, xs, round_acts=True)
+    P = M.unpack(p)
+    u, g = 2.0 ** -8, 1.13
+    W1, W2, W3 = (np.abs(M.bf16(P[k])) for k in ("W1", "W2", "W3"))
+    e1 = (u * np.abs(A0)) @ W1
+    e2 = (g * e1 + u * np.abs(H1)) @ W2
+    e3 = (g * e2 + u * np.abs(H2)) @ W3
+    d = np.abs(Or - O)
+    assert np.all(d <= 1.05 * e3 + 1e-12)
+
+    def var(a):                                   # a = m 2^e, m in [0.5, 1): ulp = 2^(e - 8)
+        _, e = np.frexp(np.asarray(a, np.float32))
+        return np.where(a != 0, np.ldexp(1.0, e - 8) ** 2 / 12.0, 0.0)
+
+    v1 = var(A0) @ W1 ** 2
+    v2 = (M.gelu_d(Z1) ** 2 * v1 + var(H1)) @ W2 ** 2
+    v3 = (M.gelu_d(Z2) ** 2 * v2 + var(H2)) @ W3 ** 2
+    ratio = np.sqrt(np.mean(d ** 2) / np.mean(v3))
+    print(f"activation rounding: measured / estimated RMS {ratio:.3f}")
+    assert 0.7 < ratio < 1.3
+    # bf16-exact features, identity scaling, zero hidden weights: the only
+    # rounded values are already representable, so the outputs agree exactly
+    q = p.copy()
+    o = 0
+    for name, shp in M.SHAPES:
+        n = int(np.prod(shp))
+        if name in ("W2", "W3"):
+            q[o:o + n] = 0.0
+        o += n
+    Xb = M.bf16(X).astype(np.float32)
+    a, _ = M.forward(q, Xb, np.zeros(M.NF), np.ones(M.NF))
+    b, _ = M.forward(q, Xb, np.zeros(M.NF), np.ones(M.NF), round_acts=True)
+    assert np.array_equal(a, b)
