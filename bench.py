@@ -212,6 +212,47 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def closure_timing(args):
+    """Cost of the closure's corrected RK4 step against the plain p = 1 step
+    on an n^3 periodic box (n = args.closure_timing; the z-extruded vortex,
+    LegendreHex, Roe, RK4 at CFL 0.1), the network at its initial weights
+    (the cost does not depend on their values; bound off), and the network
+    kernel alone (dg_mlp_forward) on the n^3 feature rows.  One JSON line."""
+    import dg_amd
+    from dg_amd import closure, hip
+
+    n = args.closure_timing
+    K, W = args.steps, args.warmup
+    m = closure.MLP(0)
+    toml = case_toml(n, p=1)
+    res = {}
+    with dg_amd.Solver(toml) as s:
+        s.init_state_from_fcn()
+        E = s.shape[0]
+        for name, fn in (("plain", lambda k: s.take_time_step(0.0, k)),
+                         ("closure", lambda k: closure.step(s, m, 1e30, k))):
+            fn(W)
+            hip.synchronize()
+            t0 = time.perf_counter()
+            fn(K)
+            hip.synchronize()
+            res[name] = (time.perf_counter() - t0) / K
+        X = closure.features(s, s.state_device_ptr())
+        m.forward(X, E)
+        reps = max(K, 10)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            m.forward(X, E)
+        t_net = (time.perf_counter() - t0) / reps
+    m.close()
+    flops = 2.0 * (70 * 128 + 128 * 128 + 128 * 40) * E
+    print(json.dumps({"metric": "closure step cost", "config": {"workload": f"closure_timing_{n}^3_p1",
+                                                                 "elements": E, "steps": K, "warmup": W},
+                      "ms_per_rk4_step": {"plain_p1": res["plain"] * 1e3, "closure": res["closure"] * 1e3},
+                      "closure_over_plain": res["closure"] / res["plain"],
+                      "network_forward": {"ms": t_net * 1e3, "rows": E, "tflops_bf16_incl_sync": flops / t_net / 1e12}}))
+
+
 def closure_bench(args):
     """Learned closure (coarse p = 1 on a 32^3 periodic box): training rows
     along coarse runs beside FP64 p = 4 runs of seeded vortex /
@@ -288,6 +329,7 @@ def main():
     ap.add_argument("--closure", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--closure-seeds", type=int, default=16, help=argparse.SUPPRESS)
     ap.add_argument("--closure-epochs", type=int, default=6, help=argparse.SUPPRESS)
+    ap.add_argument("--closure-timing", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -304,6 +346,9 @@ def main():
         return
     if args.closure:
         closure_bench(args)
+        return
+    if args.closure_timing:
+        closure_timing(args)
         return
     ne, order, dim = args.ne, args.order, args.dim
     if dim == 2 and ne == NE:

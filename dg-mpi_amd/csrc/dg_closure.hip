@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <random>
@@ -301,6 +302,28 @@ struct InferLDS {
 
 __device__ inline bf16x4 ld_bf16x4(const uint16_t* q) { return *reinterpret_cast<const bf16x4*>(q); }
 
+// The weights' LDS image (w1, w2, w3 of InferLDS, contiguous) in global
+// memory, built once per change of the bf16 weights: element i of the image
+// is W1^T / W2^T / W3^T at (row, column) with the padding zero
+constexpr int IMG1 = NH * L1S, IMG2 = NH * L2S, IMG3 = N3 * L2S, IMG = IMG1 + IMG2 + IMG3;
+
+__global__ void pack_infer_weights(const uint16_t* __restrict__ W, uint16_t* __restrict__ img) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= IMG) return;
+    uint16_t v = 0;
+    if (i < IMG1) {
+        const int n = i / L1S, k = i % L1S;
+        if (k < NF) v = W[OFF_W1 + k * NH + n];
+    } else if (i < IMG1 + IMG2) {
+        const int n = (i - IMG1) / L2S, k = (i - IMG1) % L2S;
+        if (k < NH) v = W[OFF_W2 + k * NH + n];
+    } else {
+        const int n = (i - IMG1 - IMG2) / L2S, k = (i - IMG1 - IMG2) % L2S;
+        if (n < NO && k < NH) v = W[OFF_W3 + k * NO + n];
+    }
+    img[i] = v;
+}
+
 // acc[j] += A[16 rows][K] x W^T[16 j .. 16 j + 15][K] for j < NT
 template <int NT, int K, int LA, int LW>
 __device__ __forceinline__ void mfma_layer(const uint16_t (*A)[LA], const uint16_t (*Wt)[LW], int rw, int lane,
@@ -316,31 +339,52 @@ __device__ __forceinline__ void mfma_layer(const uint16_t (*A)[LA], const uint16
     }
 }
 
-__global__ __launch_bounds__(MT) void mlp_infer_kernel(const float* __restrict__ X, const uint16_t* __restrict__ W,
+__global__ __launch_bounds__(MT) void mlp_infer_kernel(const float* __restrict__ X,
+                                                       const uint16_t* __restrict__ Wimg,
                                                        const float* __restrict__ P, const float* __restrict__ xs,
                                                        float* __restrict__ C, double bound, long rows,
                                                        unsigned* __restrict__ count) {
     __shared__ alignas(16) InferLDS L;
     const int t = threadIdx.x, lane = t & 63, rw = 16 * (t >> 6);
     const int c = lane & 15, rq = 4 * (lane >> 4);
-    // weights, transposed; the global reads run along the output index
-    for (int i = t; i < K1 * NH; i += MT) {
-        const int k = i / NH, n = i % NH;
-        L.w1[n][k] = k < NF ? W[OFF_W1 + k * NH + n] : (uint16_t)0;
-    }
-    for (int i = t; i < NH * NH; i += MT) L.w2[i % NH][i / NH] = W[OFF_W2 + i];
-    for (int i = t; i < NH * N3; i += MT) {
-        const int k = i / N3, n = i % N3;
-        L.w3[n][k] = n < NO ? W[OFF_W3 + k * NO + n] : (uint16_t)0;
+    // the weights' LDS image, a straight 16-byte copy
+    static_assert(offsetof(InferLDS, w2) == IMG1 * 2 && offsetof(InferLDS, w3) == (IMG1 + IMG2) * 2 &&
+                      IMG * 2 % 16 == 0,
+                  "contiguous weight image");
+    {
+        constexpr int NV = IMG * 2 / 16;
+        const uint4* src = reinterpret_cast<const uint4*>(Wimg);
+        uint4* dst = reinterpret_cast<uint4*>(&L.w1[0][0]);
+        for (int i = t; i < NV; i += 4 * MT) {
+            // four loads in flight before the stores
+            const int i1 = i + MT, i2 = i + 2 * MT, i3 = i + 3 * MT;
+            const uint4 v0 = src[i];
+            const uint4 v1 = i1 < NV ? src[i1] : v0, v2 = i2 < NV ? src[i2] : v0, v3 = i3 < NV ? src[i3] : v0;
+            dst[i] = v0;
+            if (i1 < NV) dst[i1] = v1;
+            if (i2 < NV) dst[i2] = v2;
+            if (i3 < NV) dst[i3] = v3;
+        }
     }
     const long ntile = (rows + MR - 1) / MR;
     for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         const long r0 = tile * MR;
         const int nr = (int)(rows - r0 < MR ? rows - r0 : MR);
         __syncthreads();                       // the staged weights; the previous tile's a0 and o are read
-        for (int i = t; i < MR * K1; i += MT) {
-            const int r = i / K1, k = i % K1;
-            L.a0[r][k] = (r < nr && k < NF) ? to_bf16((X[(r0 + r) * NF + k] - xs[k]) * xs[NF + k]) : (uint16_t)0;
+        {
+            // the tile's features: all global loads first, then the stores
+            constexpr int PER = (MR * K1 + MT - 1) / MT;
+            float xv[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i = t + u * MT, r = i / K1, k = i % K1;
+                xv[u] = (i < MR * K1 && r < nr && k < NF) ? X[(r0 + r) * NF + k] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i = t + u * MT, r = i / K1, k = i % K1;
+                if (i < MR * K1) L.a0[r][k] = (r < nr && k < NF) ? to_bf16((xv[u] - xs[k]) * xs[NF + k]) : (uint16_t)0;
+            }
         }
         __syncthreads();
         f32x4 acc[8];
@@ -459,6 +503,8 @@ struct dg_mlp {
     hipStream_t stream = nullptr;
     float* p = nullptr;          // master parameters [NP]
     uint16_t* pb = nullptr;      // BF16 working copy
+    uint16_t* img = nullptr;     // the inference kernel's LDS image of the bf16 weights
+    bool img_stale = true;       // pb changed since img was built
     float* g = nullptr;          // gradient
     float* m1 = nullptr;         // Adam moments
     float* m2 = nullptr;
@@ -483,11 +529,17 @@ void free_acts(dg_mlp* m) {
 }
 
 // the inference kernel on a persistent grid of at most one block per CU
-void launch_infer(const dg_mlp* m, const float* X, float* C, double bound, long rows, unsigned* count,
-                  hipStream_t st) {
+// (the weight image is rebuilt first, on the same stream, when pb changed)
+void launch_infer(dg_mlp* m, const float* X, float* C, double bound, long rows, unsigned* count, hipStream_t st) {
+    if (m->img_stale) {
+        hipLaunchKernelGGL(pack_infer_weights, dim3(nblk(IMG)), dim3(TB), 0, st, m->pb, m->img);
+        hcheck(hipGetLastError(), "weight image kernel");
+        m->img_stale = false;
+    }
     const long ntile = (rows + MR - 1) / MR;
     const unsigned grid = (unsigned)(ntile < MAX_INFER_BLOCKS ? ntile : MAX_INFER_BLOCKS);
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(MT), 0, st, X, m->pb, m->p, m->xs, C, bound, rows, count);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(MT), 0, st, X, m->img, m->p, m->xs, C, bound, rows,
+                       count);
     hcheck(hipGetLastError(), "inference kernel");
 }
 
@@ -609,6 +661,7 @@ int dg_mlp_create(unsigned long long seed, dg_mlp** out) {
         hcheck(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate");
         hcheck(hipMalloc(&m->p, NP * sizeof(float)), "hipMalloc(mlp)");
         hcheck(hipMalloc(&m->pb, NP * sizeof(uint16_t)), "hipMalloc(mlp)");
+        hcheck(hipMalloc(&m->img, IMG * sizeof(uint16_t)), "hipMalloc(mlp)");
         hcheck(hipMalloc(&m->g, NP * sizeof(float)), "hipMalloc(mlp)");
         hcheck(hipMalloc(&m->m1, NP * sizeof(float)), "hipMalloc(mlp)");
         hcheck(hipMalloc(&m->m2, NP * sizeof(float)), "hipMalloc(mlp)");
@@ -629,6 +682,7 @@ int dg_mlp_create(unsigned long long seed, dg_mlp** out) {
         for (int i = 0; i < NF; ++i) xs[i] = 0.f;
         hcheck(hipMemcpy(m->xs, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
         hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, NP);
+        m->img_stale = true;
         hcheck(hipGetLastError(), "to_bf16");
         hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
     });
@@ -647,6 +701,7 @@ int dg_mlp_destroy(dg_mlp* m) {
     for (float* q : {m->p, m->g, m->m1, m->m2, m->xs})
         if (q) (void)hipFree(q);
     if (m->pb) (void)hipFree(m->pb);
+    if (m->img) (void)hipFree(m->img);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return DG_OK;
@@ -673,6 +728,7 @@ int dg_mlp_set_params(dg_mlp* m, const float* host) {
         need_ptr(host, "host");
         hcheck(hipMemcpy(m->p, host, NP * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
         hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, NP);
+        m->img_stale = true;
         hcheck(hipGetLastError(), "to_bf16");
         hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
     });
@@ -747,6 +803,7 @@ int dg_mlp_train(dg_mlp* m, const float* X_dev, const float* Y_dev, long nrows, 
             ++m->step;
             const float b1 = 0.9f, b2 = 0.999f;
             const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+            m->img_stale = true;
             hipLaunchKernelGGL(adam_step, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, m->g, m->m1, m->m2, lr,
                                b1, b2, 1e-8f, c1, c2, NP);
             hcheck(hipGetLastError(), "adam");
