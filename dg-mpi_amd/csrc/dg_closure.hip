@@ -220,14 +220,14 @@ __device__ double face_mean(const dg::KTables& T, const double* m, const double*
 
 // X[e] = own 40 coefficients, then for the faces x-, x+, y-, y+, z-, z+ the
 // jump of the face-mean state (neighbour's matching face minus own), 5
-// each; one thread per feature
+// each; one thread per feature (32-bit index arithmetic: E NF < 2^32)
 __global__ void features_kernel(const double* __restrict__ U, float* __restrict__ X,
                                 const dg::KTables* __restrict__ Tp, dg::KGeom G) {
-    const long t = (long)blockIdx.x * TB + threadIdx.x;
-    if (t >= (long)G.E * NF) return;
-    const long e = t / NF;
-    const int c = (int)(t % NF);
-    const double* u = U + e * 40;
+    const unsigned t = blockIdx.x * TB + threadIdx.x;
+    if (t >= G.E * (unsigned)NF) return;
+    const unsigned e = t / NF;
+    const int c = (int)(t - e * NF);
+    const double* u = U + (size_t)e * 40;
     if (c < 40) {
         X[t] = (float)u[c];
         return;
@@ -240,12 +240,13 @@ __global__ void features_kernel(const double* __restrict__ U, float* __restrict_
         m[jj] = 0.5 * a;
     }
     const int f = (c - 40) / 5, s = (c - 40) % 5, d = f >> 1, side = f & 1;
-    const int nx = G.ne[0], ny = G.ne[1], nz = G.ne[2];
-    int cc[3] = {(int)(e % nx), (int)((e / nx) % ny), (int)(e / ((long)nx * ny))};
-    const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
-    cc[d] = side ? (cc[d] + 1) % nd : (cc[d] + nd - 1) % nd;
-    const long en = cc[0] + (long)nx * (cc[1] + (long)ny * cc[2]);
-    X[t] = (float)(face_mean(T, m, U + en * 40, d, 1 - side, s) - face_mean(T, m, u, d, side, s));
+    const unsigned nx = G.ne[0], ny = G.ne[1], nz = G.ne[2];
+    const unsigned eyz = e / nx;
+    unsigned cc[3] = {e - eyz * nx, eyz % ny, eyz / ny};
+    const unsigned nd = d == 0 ? nx : (d == 1 ? ny : nz);
+    cc[d] = side ? (cc[d] + 1 == nd ? 0u : cc[d] + 1) : (cc[d] == 0 ? nd - 1 : cc[d] - 1);
+    const unsigned en = cc[0] + nx * (cc[1] + ny * cc[2]);
+    X[t] = (float)(face_mean(T, m, U + (size_t)en * 40, d, 1 - side, s) - face_mean(T, m, u, d, side, s));
 }
 
 // coarse state = Legendre modes (i, j, k) <= 1 of the fine state (the L2
@@ -631,6 +632,8 @@ void need_ptr(const void* p, const char* what) {
 // the coarse / fine solvers this module works with: 3D Legendre periodic boxes
 void check_box(const dg_solver* s, const char* what, int order) {
     need_ptr(s, what);
+    if ((unsigned long long)s->geo.E * NF >= (1ull << 32))
+        throw std::invalid_argument(std::string(what) + ": too many elements for the closure features");
     if (s->cfg.dim != 3 || s->geo.mapped || s->cfg.basis != dg::Basis::LegendreHex || s->geo.fp32)
         throw dg::UnsupportedError(std::string(what) + ": a 3D affine FP64 LegendreHex periodic box is required");
     if (order > 0 && s->P != order)
