@@ -474,24 +474,6 @@ __global__ void bound_kernel(float* __restrict__ C, double bound, long E, unsign
 }
 
 // RK register update with dU = dU_R + dt C (the modes of the stage kernel)
-__global__ void rk_update_kernel(int mode, double alpha, double dt, const double* __restrict__ dUR,
-                                 const float* __restrict__ C, const double* __restrict__ U0, double* __restrict__ acc,
-                                 double* __restrict__ Uout, long n) {
-    const long t = (long)blockIdx.x * TB + threadIdx.x;
-    if (t >= n) return;
-    const double dU = dUR[t] + dt * (double)C[t];
-    const double u0 = U0[t];
-    if (mode == dg::MODE_RK_FIRST) {
-        Uout[t] = u0 + alpha * dU;
-        acc[t] = dU;
-    } else if (mode == dg::MODE_RK_MID) {
-        Uout[t] = u0 + alpha * dU;
-        acc[t] = acc[t] + 2.0 * dU;
-    } else {
-        Uout[t] = u0 + (1.0 / 6.0) * (acc[t] + dU);
-    }
-}
-
 thread_local std::string g_mlp_error;
 
 }  // namespace
@@ -898,36 +880,37 @@ int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int n
         hcheck(hipSetDevice(s->device), "hipSetDevice");
         const long E = s->geo.E, n = E * 40;
         float *X = nullptr, *C = nullptr;
-        double* dU = nullptr;
         unsigned* cnt = nullptr;
         hcheck(hipMalloc(&X, (size_t)E * NF * sizeof(float)), "hipMalloc");
         hcheck(hipMalloc(&C, (size_t)n * sizeof(float)), "hipMalloc");
-        hcheck(hipMalloc(&dU, (size_t)n * sizeof(double)), "hipMalloc");
         hcheck(hipMalloc(&cnt, sizeof(unsigned)), "hipMalloc");
         hcheck(hipMemset(cnt, 0, sizeof(unsigned)), "hipMemset");
         // the network's parameters are written by its own stream (training)
         hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
         auto stage = [&](const double* Ust, double* Uout, int mode, double alpha) {
-            // everything on the solver's stream: features, the fused network
-            // with its fallback, the residual, M^-1 and the update
+            // everything on the solver's stream: features, the network with
+            // its fallback, and the stage kernel whose RK epilogue takes
+            // dU = dt (M^-1 R + C)
             hipLaunchKernelGGL(features_kernel, dim3(nblk(E * NF)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
             launch_infer(m, X, C, bound, E, cnt, s->stream);
             dg::StageArgs a{};
             a.Ust = Ust;
-            a.R = s->Ub;
-            a.mode = dg::MODE_RESIDUAL;
-            hcheck(dg::launch_stage(s->P, (int)s->cfg.flux, a, s->dtab, s->geo, s->stream), "stage kernel");
-            hcheck(dg::launch_inv_mass(s->P, dt, s->Ub, dU, s->dtab, s->geo, s->stream), "inverse mass kernel");
-            hipLaunchKernelGGL(rk_update_kernel, dim3(nblk(n)), dim3(TB), 0, s->stream, mode, alpha, dt, dU, C, s->U0,
-                               s->acc, Uout, n);
-            hcheck(hipGetLastError(), "closure stage");
+            a.U0 = s->U0;
+            a.Uout = Uout;
+            a.acc = s->acc;
+            a.dt = dt;
+            a.alpha = alpha;
+            a.mode = mode;
+            hcheck(dg::launch_stage_corrected(s->P, (int)s->cfg.flux, a, C, s->dtab, s->geo, s->stream),
+                   "corrected stage kernel");
         };
         for (int k = 0; k < nsteps; ++k) {
-            // stages read U0 / Ua and write Ua / U0; the residual goes to Ub
+            // the stage kernel reads the neighbours of Ust: stages ping-pong
+            // U0 -> Ua -> Ub -> Ua -> U0, as in dg_step
             stage(s->U0, s->Ua, dg::MODE_RK_FIRST, 0.5);
             s->time += dt / 2.0;
-            stage(s->Ua, s->Ua, dg::MODE_RK_MID, 0.5);
-            stage(s->Ua, s->Ua, dg::MODE_RK_MID, 1.0);
+            stage(s->Ua, s->Ub, dg::MODE_RK_MID, 0.5);
+            stage(s->Ub, s->Ua, dg::MODE_RK_MID, 1.0);
             s->time += dt / 2.0;
             stage(s->Ua, s->U0, dg::MODE_RK_LAST, 0.0);
         }
@@ -937,7 +920,6 @@ int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int n
         if (n_fallback) *n_fallback = (long)c;
         (void)hipFree(X);
         (void)hipFree(C);
-        (void)hipFree(dU);
         (void)hipFree(cnt);
     });
 }

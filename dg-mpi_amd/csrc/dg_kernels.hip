@@ -1436,8 +1436,11 @@ __device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int l
 // contractions) in single precision on FP32 copies of the state and the
 // tables; the residual is widened to FP64 before M^-1 and the RK update,
 // which read and write the FP64 registers.  Periodic box only.
-template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>()>
-__device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T, const KGeom& G, unsigned e) {
+// CORR: the RK update takes dU = dt (M^-1 R + C), C a per-DOF correction in
+// FP32 laid out like the state (the learned closure's corrected stage)
+template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>(), bool CORR = false>
+__device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T, const KGeom& G, unsigned e,
+                                           const float* __restrict__ Corr = nullptr) {
     static_assert(sizeof(R) == 8 || !MAPPED, "the FP32 form is for the periodic box");
     static_assert(TS % WAVE == 0, "whole waves");
     using Dm = Dims<P>;
@@ -1529,7 +1532,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         if (t < NB) {
             const double rv = sRd[t];
             const double sc = (!MAPPED && T.minv_diag) ? scale * lane_diag_scale<P>(LT, t / V) : scale;
-            const double dU = sc * rv;
+            double dU = sc * rv;
+            if constexpr (CORR) dU += A.dt * (double)Corr[base + t];
             const double u0 = pf.u0[r], ac = pf.ac[r];
             const double n_rk = u0 + A.alpha * dU;                 // U0 + alpha dU
             const double n_last = u0 + (1.0 / 6.0) * (ac + dU);    // U0 + (acc + dU) / 6
@@ -1545,6 +1549,13 @@ template <int P, int FLUX, bool MAPPED, class R = double>
 __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
     stage_body<P, FLUX, MAPPED, R>(A, *Tp, G, block_element(G));
+}
+
+// The closure's corrected RK stage on an affine FP64 box (RK modes only)
+template <int P, int FLUX>
+__global__ __launch_bounds__(stage_team<P>()) void stage_corr_kernel(StageArgs A, const KTables* __restrict__ Tp,
+                                                                     KGeom G, const float* __restrict__ Corr) {
+    stage_body<P, FLUX, false, double, stage_team<P>(), true>(A, *Tp, G, block_element(G), Corr);
 }
 
 // Batched stage: the elements of many independent box cases of one order
@@ -2025,6 +2036,18 @@ hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, c
     }
     DG_DISPATCH_P(P, return stage_p<PP>(flux, a, t, g, st));
     return hipSuccess;
+}
+
+hipError_t launch_stage_corrected(int P, int flux, const StageArgs& a, const float* C, const KTables* t,
+                                  const KGeom& g, hipStream_t st) {
+    if (P != 1 || g.dim != 3 || g.mapped || g.fp32 || a.mode == MODE_RESIDUAL) return hipErrorInvalidValue;
+    dim3 grid(g.E), block(stage_team<1>());
+    switch (flux) {
+    case 0: hipLaunchKernelGGL((stage_corr_kernel<1, 0>), grid, block, 0, st, a, t, g, C); break;
+    case 1: hipLaunchKernelGGL((stage_corr_kernel<1, 1>), grid, block, 0, st, a, t, g, C); break;
+    default: hipLaunchKernelGGL((stage_corr_kernel<1, 2>), grid, block, 0, st, a, t, g, C); break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_stage_batch(int P, int flux, int fp32, int variant, const BatchCase* cases,

@@ -103,6 +103,10 @@ struct BatchCase {
 // `t` points to the case's KTables in device memory.
 hipError_t launch_stage(int P, int flux, const StageArgs& a, const KTables* t, const KGeom& g,
                         hipStream_t st);
+// RK stage of an affine FP64 p = 1 box with dU = dt (M^-1 R + C): C the
+// learned closure's FP32 correction, laid out like the state.
+hipError_t launch_stage_corrected(int P, int flux, const StageArgs& a, const float* C, const KTables* t,
+                                  const KGeom& g, hipStream_t st);
 // Batched stage over the cases of one (order, basis, flux, precision)
 // group (tables t shared): nblocks = total elements of the group,
 // blk_case[q] = case of traversal position q, stage 0..3 of RK4; variant
