@@ -224,7 +224,9 @@ int dg_batch_tuning(const dg_batch* b, int* num_groups, int* order, int* team, d
  * normalised as (x - x_mean) * x_scale and predicts Y / y_scale.  Device
  * arrays are caller-owned: X [rows][70], Y [rows][40] float, states double.
  * Parameters (dg_mlp_num_params floats): W1 [70][128], b1, W2 [128][128],
- * b2, W3 [128][40], b3. */
+ * b2, W3 [128][40], b3.  Calls on one dg_mlp must not overlap (from
+ * several host threads): it owns training buffers and the inference
+ * kernel's weight image. */
 typedef struct dg_mlp dg_mlp;
 int dg_mlp_create(unsigned long long seed, dg_mlp** out);
 int dg_mlp_destroy(dg_mlp* m);
