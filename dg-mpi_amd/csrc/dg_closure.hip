@@ -220,23 +220,25 @@ __device__ double face_mean(const dg::KTables& T, const double* m, const double*
 
 // X[e] = own 40 coefficients, then for the faces x-, x+, y-, y+, z-, z+ the
 // jump of the face-mean state (neighbour's matching face minus own), 5
-// each; one thread per feature (32-bit index arithmetic: E NF < 2^32)
+// each.  32 threads per element (32-bit index arithmetic: E 32 < 2^32):
+// all copy the 40 coefficients, threads 0..29 then take the 30 jumps
+constexpr int FT = 32;
 __global__ void features_kernel(const double* __restrict__ U, float* __restrict__ X,
                                 const dg::KTables* __restrict__ Tp, dg::KGeom G) {
     const unsigned t = blockIdx.x * TB + threadIdx.x;
-    if (t >= G.E * (unsigned)NF) return;
-    const unsigned e = t / NF;
-    const int c = (int)(t - e * NF);
+    const unsigned e = t / FT, q = t % FT;
+    if (e >= G.E) return;
     const double* u = U + (size_t)e * 40;
-    if (c < 40) {
-        X[t] = (float)u[c];
-        return;
-    }
+    float* x = X + (size_t)e * NF;
+    x[q] = (float)u[q];
+    if (q < 40 - FT) x[FT + q] = (float)u[FT + q];
+    if (q >= 30) return;
+    const int c = 40 + (int)q;
     const dg::KTables& T = *Tp;
     double m[2];
     for (int jj = 0; jj < 2; ++jj) {
         double a = 0.;
-        for (int q = 0; q < 3; ++q) a += T.w[q] * T.B[q][jj];
+        for (int g = 0; g < 3; ++g) a += T.w[g] * T.B[g][jj];
         m[jj] = 0.5 * a;
     }
     const int f = (c - 40) / 5, s = (c - 40) % 5, d = f >> 1, side = f & 1;
@@ -246,7 +248,7 @@ __global__ void features_kernel(const double* __restrict__ U, float* __restrict_
     const unsigned nd = d == 0 ? nx : (d == 1 ? ny : nz);
     cc[d] = side ? (cc[d] + 1 == nd ? 0u : cc[d] + 1) : (cc[d] == 0 ? nd - 1 : cc[d] - 1);
     const unsigned en = cc[0] + nx * (cc[1] + ny * cc[2]);
-    X[t] = (float)(face_mean(T, m, U + (size_t)en * 40, d, 1 - side, s) - face_mean(T, m, u, d, side, s));
+    x[c] = (float)(face_mean(T, m, U + (size_t)en * 40, d, 1 - side, s) - face_mean(T, m, u, d, side, s));
 }
 
 // coarse state = Legendre modes (i, j, k) <= 1 of the fine state (the L2
@@ -804,7 +806,7 @@ int dg_closure_features(dg_solver* s, const double* U_dev, float* X_dev) {
         need_ptr(U_dev, "U");
         need_ptr(X_dev, "X");
         hcheck(hipSetDevice(s->device), "hipSetDevice");
-        hipLaunchKernelGGL(features_kernel, dim3(nblk((long)s->geo.E * NF)), dim3(TB), 0, s->stream, U_dev, X_dev,
+        hipLaunchKernelGGL(features_kernel, dim3(nblk((long)s->geo.E * FT)), dim3(TB), 0, s->stream, U_dev, X_dev,
                            s->dtab, s->geo);
         hcheck(hipGetLastError(), "features kernel");
         hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
@@ -891,7 +893,7 @@ int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int n
             // everything on the solver's stream: features, the network with
             // its fallback, and the stage kernel whose RK epilogue takes
             // dU = dt (M^-1 R + C)
-            hipLaunchKernelGGL(features_kernel, dim3(nblk(E * NF)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
+            hipLaunchKernelGGL(features_kernel, dim3(nblk(E * FT)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
             launch_infer(m, X, C, bound, E, cnt, s->stream);
             dg::StageArgs a{};
             a.Ust = Ust;
