@@ -144,8 +144,9 @@ def test_targets_against_the_oracle(gpu):
         assert np.max(np.abs(Y - ref)) < 1e-6 * np.max(np.abs(ref))
 
 
-def test_zero_network_steps_like_plain_rk4(gpu):
-    t = dg_amd.case_toml(order=1, basis="LegendreHex", flux="HLLC", nelem=(6, 5, 4), timestep=0.02, ic=VORTEX[0],
+@pytest.mark.parametrize("flux", ["LaxFriedrichs", "HLLC", "Roe"])
+def test_zero_network_steps_like_plain_rk4(gpu, flux):
+    t = dg_amd.case_toml(order=1, basis="LegendreHex", flux=flux, nelem=(6, 5, 4), timestep=0.02, ic=VORTEX[0],
                          ic_data=VORTEX[1])
     m = closure.MLP(1)
     m.set_params(np.zeros(closure.num_params(), np.float32))
