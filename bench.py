@@ -17,6 +17,9 @@ and volume work of the stage kernel in FP32, RK registers, M^-1 and the L2
 reductions in FP64): its rate, and its L2 error against the FP64 run's.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Other workloads, one JSON line each: --batch 256 (batched uneven cases),
+--closure 32 (learned closure: training and held-out evaluation),
+--closure-timing 32 (cost of the corrected step against the plain one).
 """
 import argparse
 import json
