@@ -12,9 +12,16 @@ the stream the stage kernels are launched on.
 
 value = DOF-updates per second per RK stage = E (p+1)^3 / t_stage.
 
-Beside it, "fp32_form": the same run with Numerics.precision = "FP32" (face
-and volume work of the stage kernel in FP32, RK registers, M^-1 and the L2
-reductions in FP64): its rate, and its L2 error against the FP64 run's.
+Accuracy: after the timed steps the run continues to 50 RK4 steps in all
+(the configuration's count) and reports the L2 error against the exact
+advected vortex.  Beside it, "fp32_form": the same run with
+Numerics.precision = "FP32" (face and volume work of the stage kernel in
+FP32, RK registers, M^-1 and the L2 reductions in FP64): its rate, and its
+L2 error against the FP64 run's; "gaussian_bump": the configuration's second
+input (a density bump in uniform flow along a seeded direction), both
+forms, 50 steps; "l2_err_vs_fp64_oracle": one RK4 step of both forms on a
+32^3 box of the same case against the FP64 CPU oracle from the same state
+(computed by the CPU-baseline leg, which times the oracle on that box).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 Other workloads, one JSON line each: --batch 256 (batched uneven cases),
@@ -37,6 +44,8 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 # (3 + 5 + 5 + 4) / 4 = 4.25 passes x 5 variables x 8 B
 ALG_BYTES_PER_DOF = 4.25 * 5 * 8
 P, NE = 3, 64
+ORACLE_NE = 32                 # box of the oracle comparison and the CPU baseline sample
+ACC_STEPS = 50                 # the configuration's step count: accuracy figures after 50 RK4 steps
 
 
 def alg_flops_per_element(p):
@@ -72,30 +81,79 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(budget_s=20.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def legendre_l2(dU, ne, p, lo=-5.0, hi=5.0):
+    """Volume-normalised L2 norm per variable of a LegendreHex coefficient
+    difference dU[E][(p+1)^3][5] on the ne^3 box: for Legendre polynomials
+    the integral of (sum c_b phi_b)^2 over an element is |J| sum_b c_b^2
+    prod_d 2 / (2 i_d + 1), so the norm is exact for the polynomials."""
+    import numpy as np
+    n = p + 1
+    i = np.arange(n)
+    w1 = 2.0 / (2 * i + 1)
+    wb = (w1[None, None, :] * w1[None, :, None] * w1[:, None, None]).ravel()     # b = i + n j + n^2 k
+    h = (hi - lo) / ne
+    jdet, vol = (h / 2) ** 3, (hi - lo) ** 3
+    return np.sqrt(np.einsum("ebs,b->s", dU * dU, wb) * jdet / vol)
+
+
+def cpu_baseline(gpu_step=None, budget_s=20.0):
     """The CPU oracle (reference algorithm restated, dense per-element
     GEMMs, OpenMP over elements) on a bounded sample of the workload: the
-    same case on 24^3 elements, p=3, Roe, RK4; RK4 steps until ~budget_s."""
+    same case on 32^3 elements, p=3, Roe, RK4, RK4 steps until ~budget_s.
+    Its first step also checks the GPU: gpu_step = (U0, dt, {form: U1}) of
+    one GPU RK4 step at 32^3 from U0; the difference of each form's U1 to the
+    oracle's is returned as "L2 err vs FP64 oracle"."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import oracle as O
-    ne = 24
+    ne = ORACLE_NE
     prm = O.make_params(order=P, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne), lo=(-5, -5, -5),
                         hi=(5, 5, 5), gamma=1.4, R=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1, 1, 1, 1, 5))
     s = O.Solver(prm)
-    s.init_state()
-    dt = 0.1 * (10.0 / ne) / s.max_wavespeed()
+    check = None
+    if gpu_step is not None:
+        U0, dt, U1 = gpu_step
+        s.set_state(U0)
+    else:
+        s.init_state()
+        dt = 0.1 * (10.0 / ne) / s.max_wavespeed()
     steps, t0 = 0, time.perf_counter()
     while True:
         s.step("RK4", dt, 1)
         steps += 1
+        if steps == 1 and gpu_step is not None:
+            t_pause = time.perf_counter()
+            ref = s.get_state()
+            scale = float(np.max(np.abs(ref)))
+            check = {"workload": f"one RK4 step of the same case at {ne}^3 (p=3 LegendreHex, Roe, dt of CFL 0.1) "
+                                 "from the same initial state, GPU form vs the FP64 oracle",
+                     "norm": "volume-normalised L2 norm of the state difference per variable (rho, rho u, rho v, "
+                             "rho w, rho E), and max |difference| / max |oracle state|"}
+            for form, U in U1.items():
+                check[form] = {"l2": [float(x) for x in legendre_l2(U - ref, ne, P)],
+                               "max_rel": float(np.max(np.abs(U - ref)) / scale)}
+            del ref
+            t0 += time.perf_counter() - t_pause       # the comparison is not CPU-baseline time
         el = time.perf_counter() - t0
         if el > budget_s or steps >= 50:
             break
     dof = ne ** 3 * (P + 1) ** 3
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": dof * 4 * steps / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (dense reference formulation, OpenMP) on 24^3 elements of the same case "
-                      f"(p=3 LegendreHex, Roe, RK4), {steps} RK4 steps = {4 * steps} stages in {el:.1f} s"}
+    base = {"value": dof * 4 * steps / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"oracle (dense reference formulation, OpenMP, {threads} threads) on {ne}^3 elements of the "
+                      f"same case (p=3 LegendreHex, Roe, RK4), {steps} RK4 steps = {4 * steps} stages in {el:.1f} s"}
+    return base, check
 
 
 def io_bench(s, path):
@@ -388,8 +446,17 @@ def main():
     dt = s.timestep
     t_stage = ms_per_step / 4 * 1e-3
     value = dof / t_stage
+    headline = dim == 3 and not args.warp and order == P and ne == NE
 
-    # accuracy of the timed run: L2 error vs the exact advected vortex
+    def to_acc_steps(sv):
+        # the configuration's 50 RK4 steps in all (warm-up and timed steps count)
+        done = args.warmup + args.steps
+        if done < ACC_STEPS:
+            sv.take_time_step(sv.timestep, ACC_STEPS - done)
+        return max(done, ACC_STEPS)
+
+    nacc = to_acc_steps(s)
+    # accuracy of the run: L2 error vs the exact advected vortex after nacc steps
     err = s.get_error()
     finite = bool(np.all(np.isfinite(err)))
 
@@ -418,8 +485,7 @@ def main():
                    "domain": "[-5,5]^3" if dim == 3 else "[-5,5]^2"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if (order == P and ne == NE and not args.warp and dim == 3)
-                     else None},
+                     "traffic": measured_traffic() if headline else None},
         "kernel": {"name": f"stage_kernel<{order},Roe{',mapped' if args.warp else ''}{',fp32' if args.fp32 else ''}>"
                    if dim == 3 else
                    f"stage2d_kernel<{order},Roe>",
@@ -429,14 +495,17 @@ def main():
                    "fp64_tflops": flops_launch / t_stage / 1e12 if flops_launch else None,
                    "fp64_frac": flops_launch / t_stage / 1e12 / FP64_PEAK_TFLOPS if flops_launch else None},
         "l2_error_vs_exact": [float(x) for x in err],
+        "l2_error_steps": nacc,
         "finite": finite,
         "wall_s": wall,
     }
+    gpu_step = None
     if dim == 3 and not args.warp and not args.fp32 and not args.no_fp32:
         # the FP32 form on the same workload, against this FP64 run
         U64 = s.get_state()
         s.close()
         s32, ms32, _ = timed_run("FP32")
+        to_acc_steps(s32)
         err32 = s32.get_error()
         U32 = s32.get_state()
         diff = float(np.max(np.abs(U32 - U64)) / np.max(np.abs(U64)))
@@ -455,9 +524,51 @@ def main():
         s32.close()
     else:
         s.close()
+    if headline and not args.fp32 and not args.no_fp32:
+        # the second input of the configuration: a Gaussian density bump
+        # (amplitude 0.2, width 1) in uniform flow of speed 1 along a seeded
+        # random direction (numpy default_rng(2024), a normalised Gaussian
+        # vector), p = 1, both forms, 50 steps; L2 error vs exact advection
+        v = np.random.default_rng(2024).standard_normal(3)
+        v /= np.linalg.norm(v)
+        bump = {"data": [0.2, 1.0, float(v[0]), float(v[1]), float(v[2]), 1.0, 0.0, 0.0, 0.0]}
+        for form, prec in (("fp64", None), ("fp32", "FP32")):
+            sb = dg_amd.Solver(dg_amd.case_toml(order=P, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne),
+                                                lower=(-5., -5., -5.), upper=(5., 5., 5.), stepper="RK4", cfl=0.1,
+                                                end=1.0, ic="GaussianBump", ic_data=bump["data"], precision=prec))
+            sb.set_stream(stream.handle)
+            sb.init_state_from_fcn()
+            e0 = sb.get_error()
+            ev0, ev1 = hip.Event(), hip.Event()
+            ev0.record(stream)
+            sb.take_time_step(sb.timestep, ACC_STEPS)
+            ev1.record(stream)
+            ev1.synchronize()
+            bump[form] = {"ms_per_step": ev0.elapsed_ms(ev1) / ACC_STEPS,
+                          "value": dof / (ev0.elapsed_ms(ev1) / ACC_STEPS / 4 * 1e-3),
+                          "l2_error_vs_exact_t0": [float(x) for x in e0],
+                          "l2_error_vs_exact": [float(x) for x in sb.get_error()], "steps": ACC_STEPS}
+            sb.close()
+        bump["l2_error_ratio_fp32_vs_fp64"] = [float(a / b) for a, b in zip(bump["fp32"]["l2_error_vs_exact"],
+                                                                          bump["fp64"]["l2_error_vs_exact"])]
+        out["gaussian_bump"] = bump
+    if headline and not args.no_cpu:
+        # one RK4 step of both forms on the oracle's box, compared by the
+        # CPU-baseline leg with the oracle's step from the same state
+        U1 = {}
+        for form, prec in (("fp64", None), ("fp32", "FP32")):
+            so = dg_amd.Solver(case_toml(ORACLE_NE, P, precision=prec))
+            so.init_state_from_fcn()
+            U0o, dto = so.get_state(), so.timestep
+            so.take_time_step(dto, 1)
+            U1[form] = so.get_state()
+            so.close()
+        gpu_step = (U0o, dto, U1)
     stream.destroy()
     if not args.no_cpu and dim == 3:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"], check = cpu_baseline(gpu_step)
+        if check is not None:
+            out["l2_err_vs_fp64_oracle"] = check
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -346,7 +346,7 @@ __global__ __launch_bounds__(MT) void mlp_infer_kernel(const float* __restrict__
                                                        const uint16_t* __restrict__ Wimg,
                                                        const float* __restrict__ P, const float* __restrict__ xs,
                                                        float* __restrict__ C, double bound, long rows,
-                                                       unsigned* __restrict__ count) {
+                                                       unsigned long long* __restrict__ count) {
     __shared__ alignas(16) InferLDS L;
     const int t = threadIdx.x, lane = t & 63, rw = 16 * (t >> 6);
     const int c = lane & 15, rq = 4 * (lane >> 4);
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(MT) void mlp_infer_kernel(const float* __restrict__
             for (int i = 0; i < NO; ++i) n2 += (double)L.o[t][i] * L.o[t][i];
             if (t < nr && !(sqrt(n2) <= bound)) {
                 for (int i = 0; i < NO; ++i) L.o[t][i] = 0.f;
-                atomicAdd(count, 1u);
+                atomicAdd(count, 1ull);
             }
         }
         __syncthreads();
@@ -462,20 +462,9 @@ __global__ void target_at_kernel(const double* __restrict__ Df, const double* __
     Y[t] = (float)(Df[f] - Dc[t] + (Uf[f] - Uc[t]) * rtau);
 }
 
-// fallback: elements whose correction norm exceeds the bound get none
-__global__ void bound_kernel(float* __restrict__ C, double bound, long E, unsigned* __restrict__ count) {
-    const long e = (long)blockIdx.x * TB + threadIdx.x;
-    if (e >= E) return;
-    float* c = C + e * NO;
-    double n2 = 0.;
-    for (int i = 0; i < NO; ++i) n2 += (double)c[i] * c[i];
-    if (!(sqrt(n2) <= bound)) {
-        for (int i = 0; i < NO; ++i) c[i] = 0.f;
-        atomicAdd(count, 1u);
-    }
-}
+// one thread clears *p (a vector store: the address is lane-indexed)
+__global__ void clear_u64_kernel(unsigned long long* p) { p[threadIdx.x] = 0ull; }
 
-// RK register update with dU = dU_R + dt C (the modes of the stage kernel)
 thread_local std::string g_mlp_error;
 
 }  // namespace
@@ -515,7 +504,8 @@ void free_acts(dg_mlp* m) {
 
 // the inference kernel on a persistent grid of at most one block per CU
 // (the weight image is rebuilt first, on the same stream, when pb changed)
-void launch_infer(dg_mlp* m, const float* X, float* C, double bound, long rows, unsigned* count, hipStream_t st) {
+void launch_infer(dg_mlp* m, const float* X, float* C, double bound, long rows, unsigned long long* count,
+                  hipStream_t st) {
     if (m->img_stale) {
         hipLaunchKernelGGL(pack_infer_weights, dim3(nblk(IMG)), dim3(TB), 0, st, m->pb, m->img);
         hcheck(hipGetLastError(), "weight image kernel");
@@ -634,6 +624,62 @@ void tendency(dg_solver* s, const double* U, double* D) {
     hcheck(dg::launch_inv_mass(s->P, 1.0, s->Ua, D, s->dtab, s->geo, s->stream), "inverse mass kernel");
 }
 
+// argument checks of the closure step; returns the time step
+double closure_check(dg_solver* s, dg_mlp* m, double dt_step, int nsteps) {
+    check_box(s, "solver", 1);
+    need_ptr(m, "mlp");
+    if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
+    const double dt = dt_step > 0.0 ? dt_step : s->dt;
+    if (!(dt > 0.0)) throw std::invalid_argument("no time step");
+    if (s->cfg.stepper != dg::Stepper::RK4) throw dg::UnsupportedError("the closure steps RK4 only");
+    hcheck(hipSetDevice(s->device), "hipSetDevice");
+    return dt;
+}
+
+// nsteps corrected RK4 steps with the given feature rows X [E][NF],
+// corrections C [E][NO] and fallback counter cnt (device memory)
+void closure_steps(dg_solver* s, dg_mlp* m, double dt, double bound, int nsteps, float* X, float* C,
+                   unsigned long long* cnt, long* n_fallback) {
+    const long E = s->geo.E;
+    // the counter is cleared by a kernel (no memset node if the caller
+    // captures the solver's stream)
+    hipLaunchKernelGGL(clear_u64_kernel, dim3(1), dim3(1), 0, s->stream, cnt);
+    hcheck(hipGetLastError(), "clear kernel");
+    // the network's parameters are written by its own stream (training)
+    hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+    auto stage = [&](const double* Ust, double* Uout, int mode, double alpha) {
+        // everything on the solver's stream: features, the network with
+        // its fallback, and the stage kernel whose RK epilogue takes
+        // dU = dt (M^-1 R + C)
+        hipLaunchKernelGGL(features_kernel, dim3(nblk(E * FT)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
+        launch_infer(m, X, C, bound, E, cnt, s->stream);
+        dg::StageArgs a{};
+        a.Ust = Ust;
+        a.U0 = s->U0;
+        a.Uout = Uout;
+        a.acc = s->acc;
+        a.dt = dt;
+        a.alpha = alpha;
+        a.mode = mode;
+        hcheck(dg::launch_stage_corrected(s->P, (int)s->cfg.flux, a, C, s->dtab, s->geo, s->stream),
+               "corrected stage kernel");
+    };
+    for (int k = 0; k < nsteps; ++k) {
+        // the stage kernel reads the neighbours of Ust: stages ping-pong
+        // U0 -> Ua -> Ub -> Ua -> U0, as in dg_step
+        stage(s->U0, s->Ua, dg::MODE_RK_FIRST, 0.5);
+        s->time += dt / 2.0;
+        stage(s->Ua, s->Ub, dg::MODE_RK_MID, 0.5);
+        stage(s->Ub, s->Ua, dg::MODE_RK_MID, 1.0);
+        s->time += dt / 2.0;
+        stage(s->Ua, s->U0, dg::MODE_RK_LAST, 0.0);
+    }
+    unsigned long long c = 0;
+    hcheck(hipMemcpyAsync(&c, cnt, sizeof(c), hipMemcpyDeviceToHost, s->stream), "hipMemcpyAsync");
+    hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    if (n_fallback) *n_fallback = (long)c;
+}
+
 }  // namespace
 
 extern "C" {
@@ -667,6 +713,7 @@ int dg_mlp_create(unsigned long long seed, dg_mlp** out) {
         hcheck(hipMemcpy(m->m2, z.data(), NP * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
         std::vector<float> xs(2 * NF + NO, 1.f);
         for (int i = 0; i < NF; ++i) xs[i] = 0.f;
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
         hcheck(hipMemcpy(m->xs, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
         hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, NP);
         m->img_stale = true;
@@ -703,7 +750,8 @@ int dg_mlp_num_params(long* n) {
 }
 
 int dg_mlp_get_params(const dg_mlp* m, float* host) {
-    return mlp_guard(const_cast<dg_mlp*>(m)->err, [&] {
+    return mlp_guard(m ? const_cast<dg_mlp*>(m)->err : g_mlp_error, [&] {
+        need_ptr(m, "mlp");
         need_ptr(host, "host");
         hcheck(hipMemcpy(host, m->p, NP * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
     });
@@ -713,6 +761,8 @@ int dg_mlp_set_params(dg_mlp* m, const float* host) {
     return mlp_guard(m ? m->err : g_mlp_error, [&] {
         need_ptr(m, "mlp");
         need_ptr(host, "host");
+        // work already queued on the network's stream may read the parameters
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
         hcheck(hipMemcpy(m->p, host, NP * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
         hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(NP)), dim3(TB), 0, m->stream, m->p, m->pb, NP);
         m->img_stale = true;
@@ -736,6 +786,7 @@ int dg_mlp_set_scaling(dg_mlp* m, const float* x_mean, const float* x_scale, con
             if (!(y_scale[i] > 0.f)) throw std::invalid_argument("y_scale entries must be > 0");
             xs[2 * NF + i] = y_scale[i];
         }
+        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
         hcheck(hipMemcpy(m->xs, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
     });
 }
@@ -775,6 +826,7 @@ int dg_mlp_train(dg_mlp* m, const float* X_dev, const float* Y_dev, long nrows, 
         need_ptr(Y_dev, "Y");
         need_ptr(idx_dev, "idx");
         if (nrows < 1 || nidx < 1 || batch < 1) throw std::invalid_argument("empty training set or batch");
+        if (nidx < batch) throw std::invalid_argument("fewer row indices than one mini-batch (nidx < batch)");
         hcheck(hipSetDevice(m->device), "hipSetDevice");
         ensure_acts(m, batch);
         double sum = 0.;
@@ -873,56 +925,14 @@ int dg_closure_target_at(dg_solver* fine, const double* Uf_dev, dg_solver* coars
 
 int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int nsteps, long* n_fallback) {
     return mlp_guard(g_mlp_error, [&] {
-        check_box(s, "solver", 1);
-        need_ptr(m, "mlp");
-        if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
-        const double dt = dt_step > 0.0 ? dt_step : s->dt;
-        if (!(dt > 0.0)) throw std::invalid_argument("no time step");
-        if (s->cfg.stepper != dg::Stepper::RK4) throw dg::UnsupportedError("the closure steps RK4 only");
-        hcheck(hipSetDevice(s->device), "hipSetDevice");
-        const long E = s->geo.E, n = E * 40;
-        float *X = nullptr, *C = nullptr;
-        unsigned* cnt = nullptr;
-        hcheck(hipMalloc(&X, (size_t)E * NF * sizeof(float)), "hipMalloc");
-        hcheck(hipMalloc(&C, (size_t)n * sizeof(float)), "hipMalloc");
-        hcheck(hipMalloc(&cnt, sizeof(unsigned)), "hipMalloc");
-        hcheck(hipMemset(cnt, 0, sizeof(unsigned)), "hipMemset");
-        // the network's parameters are written by its own stream (training)
-        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
-        auto stage = [&](const double* Ust, double* Uout, int mode, double alpha) {
-            // everything on the solver's stream: features, the network with
-            // its fallback, and the stage kernel whose RK epilogue takes
-            // dU = dt (M^-1 R + C)
-            hipLaunchKernelGGL(features_kernel, dim3(nblk(E * FT)), dim3(TB), 0, s->stream, Ust, X, s->dtab, s->geo);
-            launch_infer(m, X, C, bound, E, cnt, s->stream);
-            dg::StageArgs a{};
-            a.Ust = Ust;
-            a.U0 = s->U0;
-            a.Uout = Uout;
-            a.acc = s->acc;
-            a.dt = dt;
-            a.alpha = alpha;
-            a.mode = mode;
-            hcheck(dg::launch_stage_corrected(s->P, (int)s->cfg.flux, a, C, s->dtab, s->geo, s->stream),
-                   "corrected stage kernel");
-        };
-        for (int k = 0; k < nsteps; ++k) {
-            // the stage kernel reads the neighbours of Ust: stages ping-pong
-            // U0 -> Ua -> Ub -> Ua -> U0, as in dg_step
-            stage(s->U0, s->Ua, dg::MODE_RK_FIRST, 0.5);
-            s->time += dt / 2.0;
-            stage(s->Ua, s->Ub, dg::MODE_RK_MID, 0.5);
-            stage(s->Ub, s->Ua, dg::MODE_RK_MID, 1.0);
-            s->time += dt / 2.0;
-            stage(s->Ua, s->U0, dg::MODE_RK_LAST, 0.0);
-        }
-        hcheck(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
-        unsigned c = 0;
-        hcheck(hipMemcpy(&c, cnt, sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy");
-        if (n_fallback) *n_fallback = (long)c;
-        (void)hipFree(X);
-        (void)hipFree(C);
-        (void)hipFree(cnt);
+        const double dt = closure_check(s, m, dt_step, nsteps);
+        const long E = s->geo.E;
+        // the feature rows, corrections and counter live on the solver
+        // handle from the first call on (freed by dg_destroy)
+        if (!s->cl_X) hcheck(hipMalloc(&s->cl_X, (size_t)E * NF * sizeof(float)), "hipMalloc");
+        if (!s->cl_C) hcheck(hipMalloc(&s->cl_C, (size_t)E * NO * sizeof(float)), "hipMalloc");
+        if (!s->cl_cnt) hcheck(hipMalloc(&s->cl_cnt, sizeof(unsigned long long)), "hipMalloc");
+        closure_steps(s, m, dt, bound, nsteps, s->cl_X, s->cl_C, s->cl_cnt, n_fallback);
     });
 }
 

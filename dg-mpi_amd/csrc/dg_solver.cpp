@@ -149,6 +149,11 @@ void free_all(dg_solver* s) {
     s->nev = nullptr;
     if (s->dtab) (void)hipFree(s->dtab);
     s->dtab = nullptr;
+    if (s->cl_X) (void)hipFree(s->cl_X);
+    if (s->cl_C) (void)hipFree(s->cl_C);
+    if (s->cl_cnt) (void)hipFree(s->cl_cnt);
+    s->cl_X = s->cl_C = nullptr;
+    s->cl_cnt = nullptr;
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
     s->U0 = s->Ua = s->Ub = s->acc = s->work = s->X = s->imm = nullptr;
     s->stream = s->own_stream = nullptr;
@@ -409,6 +414,9 @@ int dg_set_state(dg_solver* s, const double* host, int layout) {
         need(s, "solver");
         need(host, "host");
         set_device(s);
+        // stream-ordered work queued on the handle's stream (a residual of
+        // the state buffer, a step) finishes before U0 is overwritten
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
         if (layout == DG_LAYOUT_RIGHT) {
             hip_check(hipMemcpy(s->U0, host, s->ncoef * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
         } else if (layout == DG_LAYOUT_LEFT) {

@@ -33,6 +33,11 @@ struct dg_solver {
     unsigned* frec = nullptr;          // general path: face records [E][6][2]
     double* nev = nullptr;             // general path: normal evaluation points
     dg::KTables* dtab = nullptr;       // device copy of tab
+    // learned-closure stepping (dg_closure_step): feature rows [E][70],
+    // corrections [E][40] and the fallback counter, allocated on first use
+    float* cl_X = nullptr;
+    float* cl_C = nullptr;
+    unsigned long long* cl_cnt = nullptr;
     std::vector<double> mesh_xyz;      // given mesh: node coordinates [n][dim]
     std::vector<long> mesh_e2n;        // given mesh: element nodes [E][8] (2D: [E][4])
     std::string err;

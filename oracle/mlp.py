@@ -11,7 +11,9 @@ inference kernel (dg_mlp_forward and the closure step) feeds the matrix
 cores; training keeps FP32 activations.  Computed in float64 here, so the
 GPU's FP32 accumulation differs by rounding only.  Pinned by finite
 differences of the loss and by the size of the activation rounding
-(tests/test_oracle_mlp.py)."""
+(tests/test_oracle_mlp.py); GELU against known values of x Phi(x) and
+torch's exact GELU; the Adam update (adam) against its closed forms and
+torch.optim.Adam."""
 import numpy as np
 from scipy.special import erf
 
@@ -71,3 +73,15 @@ def loss_and_grad(p, X, Y, x_mean, x_scale, y_scale, round_weights=True):
     G1 = (G2 @ r(P["W2"]).T) * gelu_d(Z1)
     g["W1"], g["b1"] = A0.T @ G1, G1.sum(0)
     return loss, np.concatenate([g[name].ravel() for name, _ in SHAPES])
+
+
+def adam(p, g, m1, m2, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) of parameters p with
+    gradient g and moments m1, m2 after `step` updates (step >= 1 is this
+    update's count), in float64.  Returns (p, m1, m2)."""
+    g = np.asarray(g, np.float64)
+    m1 = b1 * np.asarray(m1, np.float64) + (1.0 - b1) * g
+    m2 = b2 * np.asarray(m2, np.float64) + (1.0 - b2) * g * g
+    mhat = m1 / (1.0 - b1 ** step)
+    vhat = m2 / (1.0 - b2 ** step)
+    return np.asarray(p, np.float64) - lr * mhat / (np.sqrt(vhat) + eps), m1, m2

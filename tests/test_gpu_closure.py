@@ -74,11 +74,28 @@ def test_mlp_forward_and_gradient_against_the_oracle(gpu):
         print(f"{name}: max rel. diff {e:.2e}", flush=True)
         assert e < 1e-4, name
         o += n
-    # one Adam step moves the loss down on this batch
+    # two Adam steps of dg_mlp_train (the whole batch, no jitter) against
+    # the oracle's Adam (oracle/mlp.py adam) fed with the GPU's gradients:
+    # the parameters after each step within 1e-3 of the largest change (the
+    # FP32 storage of the parameters rounds at ~1e-5 of the change)
     idx = hip.Buffer.from_array(np.arange(rows, dtype=np.int32), np.int32)
-    l0 = m.train(Xd, Yd, rows, idx, rows, 1e-3)
-    _, l1 = m.gradient(Xd, Yd, rows)
-    assert l1 < l0
+    lr = 1e-3
+    pk, m1, m2 = p.astype(np.float64), 0.0, 0.0
+    losses = []
+    for k in (1, 2):
+        gk, lk = m.gradient(Xd, Yd, rows)
+        losses.append(lk)
+        lt = m.train(Xd, Yd, rows, idx, rows, lr)
+        assert abs(lt - lk) < 1e-6 * lk          # the step's own loss: the same batch
+        pn, m1, m2 = MO.adam(pk, gk, m1, m2, k, lr)
+        got = m.params().astype(np.float64)
+        e = np.max(np.abs(got - pn)) / np.max(np.abs(pn - pk))
+        print(f"Adam step {k}: max diff {e:.2e} of the largest parameter change", flush=True)
+        assert e < 1e-3, k
+        pk = got
+    # and the steps move the loss down on this batch
+    _, l2 = m.gradient(Xd, Yd, rows)
+    assert l2 < losses[0]
     m.close()
 
 

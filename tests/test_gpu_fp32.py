@@ -36,7 +36,7 @@ def gpu():
 
 @pytest.mark.parametrize("order,basis,flux", [(2, "LegendreHex", "Roe"), (3, "LagrangeHex", "HLLC"),
                                               (3, "LegendreHex", "LaxFriedrichs"), (1, "LegendreHex", "Roe"),
-                                              (4, "LagrangeHex", "Roe")])
+                                              (4, "LagrangeHex", "Roe"), (3, "LegendreHex", "Roe")])
 def test_fp32_residual_against_the_oracle(gpu, order, basis, flux):
     ne = (4, 3, 5)
     with dg_amd.Solver(dg_amd.case_toml(order=order, basis=basis, flux=flux, nelem=ne, timestep=0.01,

@@ -1,0 +1,100 @@
+"""The headline configuration at its own size on the GPU: 64^3 hexahedra on
+[-5,5]^3, LegendreHex p=3 (16.8M DOF x 5), Roe flux, classical RK4 at CFL
+0.1, 50 steps, in the plain FP64 form and the FP32 form, with two inputs:
+
+* the z-extruded isentropic vortex (rho=1, u=v=1, w=0, p=1, beta=5);
+* a Gaussian density bump (amplitude 0.2, width 1.0) carried by a uniform
+  flow of speed 1 in a seeded random direction (numpy default_rng(2024):
+  a normalised Gaussian vector), p = 1.
+
+Checked: finite states; mass, momentum and energy conserved (FP64: 1e-12 of
+the largest integral; FP32: 1e-6); the L2 error against the exact advected
+solution below stated bounds (derived from the IC projection error at t=0,
+which the 50 steps may at most double, plus the measured growth); the FP32
+form's L2 error within 2x the FP64 form's for rho, rho u, rho v, rho E (at
+this resolution the discretisation error, ~1e-7 in rho, is of the size of
+FP32 rounding of O(1) states, so the FP32 form cannot stay within 1.05x);
+the ratio is printed.  One RK4 step at 32^3 of the same case against the
+FP64 oracle is in test_gpu_headline_oracle_step (bar 1e-11 for FP64,
+2e-6 for FP32, relative to the largest state value)."""
+import numpy as np
+import pytest
+
+import dg_amd
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NE, P, STEPS = 64, 3, 50
+VORTEX = ("ExtrudedIsentropicVortex", (1.0, 1.0, 1.0, 1.0, 5.0))
+
+
+def bump_data(seed=2024):
+    v = np.random.default_rng(seed).standard_normal(3)
+    v /= np.linalg.norm(v)
+    return ("GaussianBump", (0.2, 1.0, float(v[0]), float(v[1]), float(v[2]), 1.0, 0.0, 0.0, 0.0))
+
+
+def toml(ic, prec, ne=NE):
+    return dg_amd.case_toml(order=P, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne), lower=(-5., -5., -5.),
+                            upper=(5., 5., 5.), stepper="RK4", cfl=0.1, end=1.0, ic=ic[0], ic_data=ic[1],
+                            precision=prec)
+
+
+def run(ic, prec):
+    with dg_amd.Solver(toml(ic, prec)) as g:
+        g.init_state_from_fcn()
+        e0 = np.array(g.get_error())
+        I0 = np.array(g.integrals())
+        g.take_time_step(g.timestep, STEPS)
+        I1 = np.array(g.integrals())
+        e = np.array(g.get_error(g.time))
+        U = g.get_state()
+        return {"U": U, "e0": e0, "e": e, "I0": I0, "I1": I1, "dt": g.timestep, "t": g.time}
+
+
+@pytest.mark.parametrize("ic", [VORTEX, bump_data()], ids=["vortex", "bump"])
+def test_headline_64cube_p3_roe_50_steps(ic):
+    print(f"case 64^3 p3 Roe {ic[0]} FP64", flush=True)
+    r64 = run(ic, None)
+    print(f"case 64^3 p3 Roe {ic[0]} FP32", flush=True)
+    r32 = run(ic, "FP32")
+    assert r32["dt"] == r64["dt"]
+    for r, tol in ((r64, 1e-12), (r32, 1e-6)):
+        assert np.all(np.isfinite(r["U"]))
+        scale = np.max(np.abs(r["I0"][:5]))
+        assert np.all(np.abs(r["I1"][:5] - r["I0"][:5]) < tol * scale), (r["I1"] - r["I0"])
+    v = [0, 1, 2, 4] if ic is VORTEX else [0, 1, 2, 3, 4]
+    ratio = r32["e"][v] / r64["e"][v]
+    diff = float(np.max(np.abs(r32["U"] - r64["U"])) / np.max(np.abs(r64["U"])))
+    print(f"{ic[0]}: t = {r64['t']:.4f}, L2 error at t=0 {r64['e0']}, FP64 {r64['e']}, FP32 {r32['e']}, "
+          f"FP32/FP64 {ratio}, state rel. diff {diff:.2e}", flush=True)
+    # the projection error at t = 0 bounds the scale: 50 steps (t ~ 0.23) of
+    # a resolved smooth solution may at most triple it
+    assert np.all(r64["e"][v] < 3.0 * r64["e0"][v] + 1e-12), (r64["e"], r64["e0"])
+    assert np.all(ratio < 2.0), ratio
+    assert diff < 1e-5
+
+
+def test_headline_oracle_step_32cube():
+    """One RK4 step of the headline case at 32^3 (dt of CFL 0.1) in both forms
+    against the FP64 oracle from the same initial state."""
+    ne = 32
+    o = O.Solver(O.make_params(order=P, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne), lo=(-5, -5, -5),
+                               hi=(5, 5, 5), ic=VORTEX[0], ic_data=VORTEX[1]))
+    res = {}
+    for prec, bar in ((None, 1e-11), ("FP32", 2e-6)):
+        with dg_amd.Solver(toml(VORTEX, prec, ne)) as g:
+            g.init_state_from_fcn()
+            U0 = g.get_state()
+            dt = g.timestep
+            g.take_time_step(dt, 1)
+            U1 = g.get_state()
+        if prec is None:
+            o.set_state(U0)
+            o.step("RK4", dt, 1)
+            ref = o.get_state()
+        e = float(np.max(np.abs(U1 - ref)) / np.max(np.abs(ref)))
+        res[prec] = e
+        print(f"32^3 p3 Roe vortex, one RK4 step, {prec or 'FP64'} vs oracle: max rel diff {e:.2e}", flush=True)
+        assert e < bar

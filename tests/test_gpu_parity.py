@@ -48,6 +48,9 @@ CASES = [
     (4, "LegendreHex", "Roe", (2, 2, 3), (-5, -5, -5), (5, 5, 5), "ExtrudedIsentropicVortex", (1, 1, 1, 1, 5)),
     (4, "LagrangeHex", "HLLC", (2, 3, 2), (-5, -5, -5), (5, 5, 5), "ExtrudedIsentropicVortex", (1, 1, 1, 1, 5)),
     (5, "LagrangeHex", "Roe", (2, 2, 2), (-5, -5, -5), (5, 5, 5), "ExtrudedIsentropicVortex", (1, 1, 1, 1, 5)),
+    # the headline configuration's discretisation (p=3 LegendreHex, Roe, the
+    # z-extruded vortex on [-5,5]^3) on a small box
+    (3, "LegendreHex", "Roe", (4, 4, 4), (-5, -5, -5), (5, 5, 5), "ExtrudedIsentropicVortex", (1, 1, 1, 1, 5)),
 ]
 
 
@@ -108,7 +111,7 @@ def test_inv_mass(case, gpu):
 
 
 @pytest.mark.parametrize("stepper", ["RK4", "FE"])
-@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], CASES[6]], ids=case_id)
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], CASES[6], CASES[9]], ids=case_id)
 def test_time_steps(case, stepper):
     dt = 0.002 if case[6] != "TaylorGreen" else 0.01
     g, o = make_pair(case, stepper=stepper, timestep=dt)

@@ -1,7 +1,10 @@
 """Pins of the closure network's CPU restatement (oracle/mlp.py): the
 backward pass against central finite differences of the loss (weights not
 rounded, so the loss is smooth), the bfloat16 rounding against values it
-must produce, and the GELU derivative against differences of GELU."""
+must produce, the GELU derivative against differences of GELU, GELU itself
+against x Phi(x) at known values of the normal CDF and against torch's exact
+GELU, and the Adam update against its closed forms (first step, constant
+gradient) and torch.optim.Adam on a varying gradient sequence."""
 import numpy as np
 import pytest
 
@@ -103,3 +106,51 @@ def test_activation_rounding_stays_within_its_propagated_bound():
     a, _ = M.forward(q, Xb, np.zeros(M.NF), np.ones(M.NF))
     b, _ = M.forward(q, Xb, np.zeros(M.NF), np.ones(M.NF), round_acts=True)
     assert np.array_equal(a, b)
+
+
+def test_gelu_known_values_and_torch():
+    # GELU(x) = x Phi(x): Phi(1) = 0.8413447460685429, Phi(2) = 0.9772498680518208,
+    # Phi(-0.5) = 0.3085375387259869 (normal CDF tables)
+    assert abs(M.gelu(np.array([1.0]))[0] - 0.8413447460685429) < 1e-15
+    assert abs(M.gelu(np.array([2.0]))[0] - 2.0 * 0.9772498680518208) < 1e-15
+    assert abs(M.gelu(np.array([-0.5]))[0] + 0.5 * 0.3085375387259869) < 1e-15
+    assert M.gelu(np.array([0.0]))[0] == 0.0
+    torch = pytest.importorskip("torch")
+    z = np.linspace(-6, 6, 241)
+    ref = torch.nn.functional.gelu(torch.tensor(z, dtype=torch.float64), approximate="none").numpy()
+    assert np.max(np.abs(M.gelu(z) - ref)) < 1e-15
+    # the tanh approximation differs from the exact form by ~1e-4: a wrong form is caught
+    tanh = torch.nn.functional.gelu(torch.tensor(z, dtype=torch.float64), approximate="tanh").numpy()
+    assert np.max(np.abs(M.gelu(z) - tanh)) > 1e-5
+
+
+def test_adam_closed_forms():
+    rng = np.random.default_rng(5)
+    p0 = rng.normal(0, 1, 50)
+    g = rng.normal(0, 1, 50)
+    lr, eps = 1e-3, 1e-8
+    # first step from zero moments: the bias-corrected moments are g and g^2
+    p1, m1, m2 = M.adam(p0, g, 0.0, 0.0, 1, lr)
+    assert np.allclose(p1, p0 - lr * g / (np.abs(g) + eps), rtol=0, atol=1e-15)
+    assert np.allclose(m1, 0.1 * g) and np.allclose(m2, 0.001 * g * g)
+    # a constant gradient keeps m_hat = g, v_hat = g^2 at every step
+    p, m1, m2 = p0, 0.0, 0.0
+    for k in range(1, 8):
+        p, m1, m2 = M.adam(p, g, m1, m2, k, lr)
+    assert np.allclose(p, p0 - 7 * lr * g / (np.abs(g) + eps), rtol=0, atol=1e-13)
+
+
+def test_adam_against_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(6)
+    p0 = rng.normal(0, 1, 40)
+    grads = [rng.normal(0, 1, 40) * s for s in (1.0, -3.0, 0.2, 5.0, 1.0)]
+    t = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([t], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    p, m1, m2 = p0, 0.0, 0.0
+    for k, g in enumerate(grads, 1):
+        opt.zero_grad()
+        t.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        p, m1, m2 = M.adam(p, g, m1, m2, k, 1e-2)
+        assert np.max(np.abs(p - t.detach().numpy())) < 1e-13, k
