@@ -26,7 +26,8 @@ forms, 50 steps; "l2_err_vs_fp64_oracle": one RK4 step of both forms on a
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 Other workloads, one JSON line each: --batch 256 (batched uneven cases),
 --closure 32 (learned closure: training and held-out evaluation),
---closure-timing 32 (cost of the corrected step against the plain one).
+--closure-timing 32 (cost of the corrected step against the plain one),
+--tgv 96 (96^3 p=4 Taylor-Green to t=1, FP64 and FP32 forms, KE/mass drift).
 """
 import argparse
 import json
@@ -273,6 +274,117 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def tgv_bench(args):
+    """The large single case: ne^3 periodic cube [0, 2 pi]^3 (ne = args.tgv,
+    96 in the configuration: 110.6M DOF x 5), LegendreHex p = 4, Roe flux,
+    inviscid Taylor-Green vortex at Mach 0.1, RK4 to t = 1 at CFL 0.03 (p = 4
+    RK4 on this flow is unstable at CFL 0.04: the oracle's run on a 6^3 box
+    blows up within 100 steps, and is stable at 0.03 and 0.02), in the plain
+    FP64 form and the FP32 form (FP32 contractions and fluxes, compensated
+    FP64 solution update).  Reported per form: the stage rate, the fraction
+    of the HBM roofline (170 B per DOF per stage), the total mass and kinetic
+    energy at 11 times, their drift from t = 0, and FP32 against FP64 at each
+    time.  The CPU-baseline leg runs the oracle on an 8^3 box of the same
+    case, timing it and comparing both forms' states after the same steps."""
+    import numpy as np
+    import dg_amd
+    from dg_amd import hip
+
+    ne, p, cfl = args.tgv, 4, 0.03
+    E, nb = ne ** 3, (p + 1) ** 3
+    dof = E * nb
+    log = lambda m: print(m, file=sys.stderr, flush=True)       # progress lines
+
+    def toml(n, prec):
+        return dg_amd.case_toml(order=p, basis="LegendreHex", flux="Roe", nelem=(n, n, n), lower=(0., 0., 0.),
+                                upper=(2 * np.pi,) * 3, stepper="RK4", cfl=cfl, end=1.0, ic="TaylorGreen",
+                                ic_data=(0.1,), precision=prec)
+
+    stream = hip.Stream()
+    forms = {}
+    for form, prec in (("fp64", None), ("fp32", "FP32")):
+        s = dg_amd.Solver(toml(ne, prec))
+        s.set_stream(stream.handle)
+        s.init_state_from_fcn()
+        dt, n = s.timestep, s.num_iterations
+        I = [s.integrals()]
+        times = [0.0]
+        ms = 0.0
+        bounds = [round(n * k / 10) for k in range(11)]
+        for k in range(10):
+            ev0, ev1 = hip.Event(), hip.Event()
+            ev0.record(stream)
+            s.take_time_step(dt, bounds[k + 1] - bounds[k])
+            ev1.record(stream)
+            ev1.synchronize()
+            ms += ev0.elapsed_ms(ev1)
+            I.append(s.integrals())
+            times.append(s.time)
+            log(f"tgv {form}: step {bounds[k + 1]} / {n}, t = {s.time:.4f}, KE = {I[-1][5]:.10e}")
+        I = np.array(I)
+        t_stage = ms / n / 4 * 1e-3
+        forms[form] = {"steps": n, "dt": dt, "t_end": s.time, "ms_per_step": ms / n,
+                       "value": dof / t_stage, "finite": bool(np.all(np.isfinite(I))),
+                       "roofline_frac": 170.0 * dof / t_stage / 1e9 / HBM_PEAK_GBS,
+                       "times": times, "mass": [float(x) for x in I[:, 0]],
+                       "kinetic_energy": [float(x) for x in I[:, 5]],
+                       "mass_drift": float(np.max(np.abs(I[:, 0] - I[0, 0])) / I[0, 0]),
+                       "energy_drift": float(np.max(np.abs(I[:, 4] - I[0, 4])) / I[0, 4]),
+                       "kinetic_energy_change": float((I[-1, 5] - I[0, 5]) / I[0, 5])}
+        s.close()
+    f64, f32 = forms["fp64"], forms["fp32"]
+    k64, k32 = np.array(f64["kinetic_energy"]), np.array(f32["kinetic_energy"])
+    m64, m32 = np.array(f64["mass"]), np.array(f32["mass"])
+    out = {
+        "metric": "DOF-updates/s per RK stage, Taylor-Green p=4 to t=1; KE and mass drift FP32 vs FP64",
+        "value": f64["value"], "unit": "DOF-updates/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (Taylor-Green vortex, generated on the GPU)",
+        "config": {"workload": f"euler3d_taylor_green_{ne}^3_p4_legendre_roe_rk4_cfl{cfl}_mach0.1_t1",
+                   "elements": E, "order": p, "dof": dof},
+        "fp64": f64, "fp32_compensated": f32,
+        "fp32_vs_fp64": {"speedup": f64["ms_per_step"] / f32["ms_per_step"],
+                         "kinetic_energy_max_rel_diff": float(np.max(np.abs(k32 - k64)) / k64[0]),
+                         "mass_max_rel_diff": float(np.max(np.abs(m32 - m64)) / m64[0])},
+    }
+    stream.destroy()
+    if not args.no_cpu:
+        out["cpu_baseline"], out["oracle_check_8cube"] = tgv_cpu_baseline(toml)
+    print(json.dumps(out))
+
+
+def tgv_cpu_baseline(toml, ne=8, steps=60):
+    """CPU leg of the Taylor-Green workload: the oracle on an ne^3 box of the
+    same case (p = 4), `steps` RK4 steps timed; both GPU forms run the same
+    steps from the same state and are compared with it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import dg_amd
+    import oracle as O
+    U1, U0, dt = {}, None, None
+    for form, prec in (("fp64", None), ("fp32", "FP32")):
+        with dg_amd.Solver(toml(ne, prec)) as g:
+            g.init_state_from_fcn()
+            U0, dt = g.get_state(), g.timestep
+            g.take_time_step(dt, steps)
+            U1[form] = g.get_state()
+    o = O.Solver(O.make_params(order=4, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne), lo=(0, 0, 0),
+                               hi=(2 * np.pi,) * 3, ic="TaylorGreen", ic_data=(0.1,)))
+    o.set_state(U0)
+    t0 = time.perf_counter()
+    o.step("RK4", dt, steps)
+    el = time.perf_counter() - t0
+    ref = o.get_state()
+    scale = float(np.max(np.abs(ref)))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    dofs = ne ** 3 * 125
+    base = {"value": dofs * 4 * steps / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"oracle on {ne}^3 elements of the same case (p=4, Roe, RK4), {steps} steps in {el:.1f} s"}
+    check = {"workload": f"{steps} RK4 steps of the case on {ne}^3 from the same state",
+             **{form: {"max_rel_diff": float(np.max(np.abs(U - ref)) / scale)} for form, U in U1.items()}}
+    return base, check
+
+
 def closure_timing(args):
     """Cost of the closure's corrected RK4 step against the plain p = 1 step
     on an n^3 periodic box (n = args.closure_timing; the z-extruded vortex,
@@ -391,6 +503,8 @@ def main():
     ap.add_argument("--closure-seeds", type=int, default=16, help=argparse.SUPPRESS)
     ap.add_argument("--closure-epochs", type=int, default=6, help=argparse.SUPPRESS)
     ap.add_argument("--closure-timing", type=int, default=0, help=argparse.SUPPRESS)
+    # the large Taylor-Green case (mesh edge; 96 in the configuration)
+    ap.add_argument("--tgv", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -410,6 +524,9 @@ def main():
         return
     if args.closure_timing:
         closure_timing(args)
+        return
+    if args.tgv:
+        tgv_bench(args)
         return
     ne, order, dim = args.ne, args.order, args.dim
     if dim == 2 and ne == NE:
@@ -528,13 +645,15 @@ def main():
         # the second input of the configuration: a Gaussian density bump
         # (amplitude 0.2, width 1) in uniform flow of speed 1 along a seeded
         # random direction (numpy default_rng(2024), a normalised Gaussian
-        # vector), p = 1, both forms, 50 steps; L2 error vs exact advection
+        # vector), p = 1, both forms, 50 steps; L2 error vs exact advection.
+        # CFL 0.06: at this Mach number (0.85) p = 3 RK4 is unstable at 0.1
+        # (the oracle blows up as well)
         v = np.random.default_rng(2024).standard_normal(3)
         v /= np.linalg.norm(v)
-        bump = {"data": [0.2, 1.0, float(v[0]), float(v[1]), float(v[2]), 1.0, 0.0, 0.0, 0.0]}
+        bump = {"data": [0.2, 1.0, float(v[0]), float(v[1]), float(v[2]), 1.0, 0.0, 0.0, 0.0], "cfl": 0.06}
         for form, prec in (("fp64", None), ("fp32", "FP32")):
             sb = dg_amd.Solver(dg_amd.case_toml(order=P, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne),
-                                                lower=(-5., -5., -5.), upper=(5., 5., 5.), stepper="RK4", cfl=0.1,
+                                                lower=(-5., -5., -5.), upper=(5., 5., 5.), stepper="RK4", cfl=0.06,
                                                 end=1.0, ic="GaussianBump", ic_data=bump["data"], precision=prec))
             sb.set_stream(stream.handle)
             sb.init_state_from_fcn()

@@ -658,6 +658,7 @@ void closure_steps(dg_solver* s, dg_mlp* m, double dt, double bound, int nsteps,
         a.U0 = s->U0;
         a.Uout = Uout;
         a.acc = s->acc;
+        a.cmp = s->cmp;
         a.dt = dt;
         a.alpha = alpha;
         a.mode = mode;

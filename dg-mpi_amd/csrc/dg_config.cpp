@@ -192,6 +192,13 @@ std::string get_str(const TomlDoc& d, const std::string& t, const std::string& k
     return v->str;
 }
 
+bool get_bool(const TomlDoc& d, const std::string& t, const std::string& k, bool dflt) {
+    const TomlValue* v = find(d, t, k);
+    if (!v) return dflt;
+    if (v->kind != TomlValue::BOOL) throw InputError(t + "." + k + " must be true or false");
+    return v->b;
+}
+
 std::vector<double> get_num_array(const TomlDoc& d, const std::string& t, const std::string& k,
                                   bool required) {
     const TomlValue* v = find(d, t, k);
@@ -277,6 +284,11 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     c.end_time = get_num(d, "Stepper", "end", true, 0.0);
     c.timestep = get_num(d, "Stepper", "timestep", false, 0.0);
     c.cfl = get_num(d, "Stepper", "cfl", false, 0.0);
+    // compensated = true (new key; the default of the reduced-precision
+    // forms): the solution update U0 + increment is carried as a pair of
+    // FP64 registers (the sum and its rounding error, TwoSum)
+    c.compensated = get_bool(d, "Stepper", "compensated", c.fp32);
+    if (c.compensated && dim != 3) throw UnsupportedError("Stepper.compensated with Physics.dim = 2 (3D only)");
     if (c.timestep <= 0.0 && c.cfl <= 0.0)
         throw InputError("missing required key Stepper.timestep (or Stepper.cfl)");
 

@@ -67,6 +67,7 @@ struct CaseConfig {
     Stepper stepper = Stepper::FE;
     double timestep = 0.0;     // 0 => derived from cfl
     double cfl = 0.0;          // new key, used when timestep is absent
+    bool compensated = false;  // new key: compensated (TwoSum) solution update
     double end_time = 0.0;
     // [Physics] (src/solver/base.cpp:34-44; src/physics/base/base.cpp:24-32)
     double gamma = 1.4;

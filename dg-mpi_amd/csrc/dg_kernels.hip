@@ -103,6 +103,17 @@ __device__ __forceinline__ R tphip(const KTables& T, int i) {
     else return T.phip[i];
 }
 
+template <class R>
+__device__ __forceinline__ R tBw(const KTables& T, int a, int i) {
+    if constexpr (sizeof(R) == 4) return T.Bw32[a][i];
+    else return T.Bw[a][i];
+}
+template <class R>
+__device__ __forceinline__ R tDw(const KTables& T, int a, int i) {
+    if constexpr (sizeof(R) == 4) return T.Dw32[a][i];
+    else return T.Dw[a][i];
+}
+
 template <int T, int TS = WAVE, class F>
 __device__ __forceinline__ void team_for(int lane, F&& f) {
 #pragma unroll
@@ -169,6 +180,20 @@ __device__ __forceinline__ float frcp(float x) {
 }
 
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Reciprocal square root of a positive normal argument: the hardware
+// estimate refined by two Newton-Raphson steps y <- y + y (1 - x y^2) / 2
+// (FP32: the hardware estimate, about an ulp).
+__device__ __forceinline__ double frsq(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * y;
+    y = fma(y, fma(-x * y, h, 0.5), y);
+    h = 0.5 * y;
+    y = fma(y, fma(-x * y, h, 0.5), y);
+    return y;
+}
+
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // src/physics/euler/functions.cpp:25-40
 __device__ __forceinline__ double max_wave_speed(double gam, const double* U) {
@@ -287,7 +312,7 @@ template <class R>
 __device__ __forceinline__ R entropy_fix(R lam, R lamL, R lamR) {
     const R delta = fmax(R(0), fmax(lam - lamL, lamR - lam));
     R a = fabs(lam);
-    if (a < delta) a = R(0.5) * (lam * lam + delta * delta) / delta;   // rare: transonic expansion
+    if (a < delta) a = R(0.5) * (lam * lam + delta * delta) * frcp(delta);   // rare: transonic expansion
     return a;
 }
 
@@ -305,16 +330,21 @@ __device__ __forceinline__ void riemann_roe(R gam, const R* UL, const R* UR, R n
     const R pL = gm1 * (UL[4] - R(0.5) * (UL[1] * uL[0] + UL[2] * uL[1] + UL[3] * uL[2]));
     const R pR = gm1 * (UR[4] - R(0.5) * (UR[1] * uR[0] + UR[2] * uR[1] + UR[3] * uR[2]));
     const R HL = (UL[4] + pL) * r1L, HR = (UR[4] + pR) * r1R;
-    const R sL = fsqrt(rL), sR = fsqrt(rR), isum = frcp(sL + sR);
+    // Roe weights sqrt(rL) / (sqrt(rL) + sqrt(rR)) and sqrt(rR) / (...) from
+    // one square root k = sqrt(rR / rL); sqrt(rL rR) = rL k
+    const R k = fsqrt(rR * r1L);
+    const R wl = frcp(R(1.) + k), wr = k * wl;
     R ut[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) ut[d] = (sL * uL[d] + sR * uR[d]) * isum;
-    const R Ht = (sL * HL + sR * HR) * isum;
+    for (int d = 0; d < 3; d++) ut[d] = fma(wl, uL[d], wr * uR[d]);
+    const R Ht = fma(wl, HL, wr * HR);
     const R q2 = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
     const R c2 = gm1 * (Ht - R(0.5) * q2);
-    const R ct = fsqrt(c2);
-    const R ic2 = frcp(c2);
-    const R rt = sL * sR;
+    // c = sqrt(c2) and 1 / c2 from one reciprocal square root
+    const R yc = frsq(c2);
+    const R ct = c2 * yc;
+    const R ic2 = yc * yc;
+    const R rt = rL * k;
     const R unt = ut[D], unL = uL[D], unR = uR[D];
     const R drho = rR - rL, dp = pR - pL, dun = unR - unL;
     const R a1 = (dp - rt * ct * dun) * (R(0.5) * ic2);
@@ -696,6 +726,28 @@ __device__ __forceinline__ unsigned brick_element(const KGeom& g, unsigned p) {
     const unsigned y = (ty << ly) | ((q >> lx) & ((1u << ly) - 1u));
     const unsigned z = (tz << lz) | (q >> lxy);
     return x + (unsigned)g.ne[0] * (y + (unsigned)g.ne[1] * z);
+}
+
+// the same traversal, also returning the element's box coordinates
+__device__ __forceinline__ unsigned brick_element_xyz(const KGeom& g, unsigned p, int* c) {
+    const unsigned lx = g.tile[0], ly = g.tile[1], lz = g.tile[2];
+    const unsigned lxy = lx + ly;
+    const unsigned q = p & ((1u << (lxy + lz)) - 1u), tp = p >> (lxy + lz);
+    const unsigned ntx = (unsigned)g.ne[0] >> lx, nty = (unsigned)g.ne[1] >> ly;
+    const unsigned tx = tp % ntx, t2 = tp / ntx, ty = t2 % nty, tz = t2 / nty;
+    const unsigned x = (tx << lx) | (q & ((1u << lx) - 1u));
+    const unsigned y = (ty << ly) | ((q >> lx) & ((1u << ly) - 1u));
+    const unsigned z = (tz << lz) | (q >> lxy);
+    c[0] = (int)x;
+    c[1] = (int)y;
+    c[2] = (int)z;
+    return x + (unsigned)g.ne[0] * (y + (unsigned)g.ne[1] * z);
+}
+
+__device__ __forceinline__ unsigned block_element_xyz(const KGeom& g, int* c) {
+    const unsigned w = blockIdx.x;
+    const unsigned p = g.xcd_remap ? (w & 7u) * (g.E >> 3) + (w >> 3) : w;
+    return brick_element_xyz(g, p, c);
 }
 
 __device__ __forceinline__ unsigned block_element(const KGeom& g) {
@@ -1172,6 +1224,351 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTabT<R>
     sync();
 }
 
+// ---------------------------------------------------------------------------
+// Affine periodic box: the same face and volume work with fewer instructions
+// per element.
+//   * Each lane's task decode (task t = q V + s, variable innermost) is done
+//     once per round at the top of the stage and handed to every step.
+//   * The (p+2)-point weights sit in the 1D factors of the projection and
+//     the lift (Bw, Dw: w_a B[a][i], w_a D[a][i]), so no weight products
+//     are formed per point; the face side's sign sits in the expansion.
+//   * The point step writes the three contravariant flux vectors of each
+//     volume point (G_d = F_d (dxi_d/dx_d) |J|, 15 values), so the first
+//     projection step is nothing but multiply-adds.
+//   * The Riemann step picks the left / right trace by address, not by
+//     per-value selects.
+// ---------------------------------------------------------------------------
+template <int P>
+struct BoxLDS {
+    using Dm = Dims<P>;
+    static constexpr int N = Dm::N, M = Dm::M;
+    static constexpr int U = Dm::NNN * V;
+    static constexpr int SLAB = 12 * Dm::NN * V;
+    static constexpr int F1 = 12 * N * M * V;
+    static constexpr int F2 = 12 * Dm::MM * V;
+    static constexpr int FH = 6 * Dm::MM * V;
+    static constexpr int L1 = 6 * N * M * V;
+    static constexpr int L2 = 6 * Dm::NN * V;
+    static constexpr int X1 = N * N * M * V;
+    static constexpr int X2 = N * M * M * V;
+    static constexpr int UQ = Dm::MMM * V;
+    static constexpr int QD = 3 * Dm::MMM * V;          // [d][c][b][a][s]
+    static constexpr int C = 3 * N * M * M * V;
+    static constexpr int BV = 2 * N * N * M * V;
+    static constexpr int A = cmax(cmax(U + cmax(cmax(F1, FH), L2), X2), cmax(QD, BV));
+    static constexpr int Bk = cmax(cmax(cmax(SLAB, F2), L1), cmax(cmax(X1, UQ), C));
+    static constexpr int R = Dm::NNN * V;
+    static constexpr int TOTAL = R + A + Bk;
+    static constexpr int RP = ((2 * Dm::MM + WAVE - 1) / WAVE) * WAVE;
+};
+
+// per-lane task decode of up to NR rounds of a team of TS threads
+template <int TS, int NR>
+struct TaskDec {
+    int s[NR], q[NR];
+    __device__ __forceinline__ explicit TaskDec(int lane) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int t = r * TS + lane;
+            q[r] = t / V;
+            s[r] = t - V * q[r];
+        }
+    }
+};
+
+template <int P, int TS>
+constexpr int box_rounds() {
+    constexpr int N = P + 1, M = P + 2;
+    constexpr int T = cmax(cmax(12 * M * V, N * N * N * V), 6 * N * N * V);
+    return (T + TS - 1) / TS;
+}
+
+// rounds over T tasks with the decoded (q, s) of each round
+template <int T, int TS, int NR, class F>
+__device__ __forceinline__ void team_qs(const TaskDec<TS, NR>& dc, int lane, F&& f) {
+    static_assert((T + TS - 1) / TS <= NR, "task decode rounds");
+#pragma unroll
+    for (int r = 0; r < (T + TS - 1) / TS; ++r) {
+        if (r * TS + lane < T) f(dc.q[r], dc.s[r], r * TS + lane);
+    }
+}
+
+template <int P, int FLUX, int TS, int NR, class R>
+__device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<R>& LT, const KGeom& G,
+                                               const TaskDec<TS, NR>& dc, int lane, const R* sU, R* sR, R* bankA,
+                                               R* bankB) {
+    using Dm = Dims<P>;
+    using L = BoxLDS<P>;
+    constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, NNN = Dm::NNN;
+    R* slab = bankB;
+    R* f1 = bankA + L::U;
+    R* f2 = bankB;
+    R* fh = bankA + L::U;
+    R* l1 = bankB;
+    R* l2 = bankA + L::U;
+
+    // 1. own slabs of both sides (the neighbour slabs are already in `slab`)
+    team_qs<3 * NN * V, TS>(dc, lane, [&](int u, int s, int) {
+        const int t1 = u % N, u2 = u / N, t2 = u2 % N, d = u2 / N;
+        R um = 0., up = 0.;
+        const int sr = d == 0 ? V : (d == 1 ? N * V : NN * V);
+        const R* src = sU + bidx_d<N>(d, 0, t1, t2) * V + s;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            const R x = src[r * sr];
+            um = fma(tphim<R>(T, r), x, um);
+            up = fma(tphip<R>(T, r), x, up);
+        }
+        slab[((d * 4 + 0) * NN + t2 * N + t1) * V + s] = um;
+        slab[((d * 4 + 1) * NN + t2 * N + t1) * V + s] = up;
+    });
+    sync();
+    // 2. interpolate along t1 to the face quadrature points
+    team_qs<12 * N * V, TS>(dc, lane, [&](int u, int s, int) {
+        const int t2 = u % N, ds = u / N;
+        R x[N];
+#pragma unroll
+        for (int t1 = 0; t1 < N; ++t1) x[t1] = slab[(ds * NN + t2 * N + t1) * V + s];
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+            R acc = 0.;
+#pragma unroll
+            for (int t1 = 0; t1 < N; ++t1) acc = fma(tB<R>(T, a, t1), x[t1], acc);
+            f1[((ds * N + t2) * M + a) * V + s] = acc;
+        }
+    });
+    sync();
+    // 3. interpolate along t2
+    team_qs<12 * M * V, TS>(dc, lane, [&](int u, int s, int) {
+        const int a1 = u % M, ds = u / M;
+        R x[N];
+#pragma unroll
+        for (int t2 = 0; t2 < N; ++t2) x[t2] = f1[((ds * N + t2) * M + a1) * V + s];
+#pragma unroll
+        for (int a2 = 0; a2 < M; ++a2) {
+            R acc = 0.;
+#pragma unroll
+            for (int t2 = 0; t2 < N; ++t2) acc = fma(tB<R>(T, a2, t2), x[t2], acc);
+            f2[((ds * M + a2) * M + a1) * V + s] = acc;
+        }
+    });
+    sync();
+    // 4. Riemann flux at the face points (the + face's element is the left
+    //    one); the weights and the side's sign are applied by the lift.
+    //    Each wave handles one direction.
+    team_for<3 * L::RP, TS>(lane, [&](int t) {
+        const int d = t / L::RP, q = t % L::RP;
+        if (q >= 2 * MM) return;
+        const int pt = q % MM, side = q / MM;
+        const R* own = f2 + ((d * 4 + side) * MM + pt) * V;
+        const R* nbv = f2 + ((d * 4 + 2 + side) * MM + pt) * V;
+        const R* pl = side ? own : nbv;
+        const R* pr = side ? nbv : own;
+        R UL[V], UR[V], F[V];
+#pragma unroll
+        for (int s = 0; s < V; ++s) {
+            UL[s] = pl[s];
+            UR[s] = pr[s];
+        }
+        if (d == 0) riemann<FLUX, 0>(R(G.gamma), UL, UR, R(G.farea[0]), F);
+        else if (d == 1) riemann<FLUX, 1>(R(G.gamma), UL, UR, R(G.farea[1]), F);
+        else riemann<FLUX, 2>(R(G.gamma), UL, UR, R(G.farea[2]), F);
+        R* o = fh + ((d * 2 + side) * MM + pt) * V;
+#pragma unroll
+        for (int s = 0; s < V; ++s) o[s] = F[s];
+    });
+    sync();
+    // 5. lift: transpose of the interpolation with the weights, along t2
+    //    then t1
+    team_qs<6 * M * V, TS>(dc, lane, [&](int u, int s, int) {
+        const int a1 = u % M, ds = u / M;
+        R f[M];
+#pragma unroll
+        for (int a2 = 0; a2 < M; ++a2) f[a2] = fh[((ds * M + a2) * M + a1) * V + s];
+#pragma unroll
+        for (int t2 = 0; t2 < N; ++t2) {
+            R acc = 0.;
+#pragma unroll
+            for (int a2 = 0; a2 < M; ++a2) acc = fma(tBw<R>(T, a2, t2), f[a2], acc);
+            l1[((ds * N + t2) * M + a1) * V + s] = acc;
+        }
+    });
+    sync();
+    team_qs<6 * N * V, TS>(dc, lane, [&](int u, int s, int) {
+        const int t2 = u % N, ds = u / N;
+        R f[M];
+#pragma unroll
+        for (int a1 = 0; a1 < M; ++a1) f[a1] = l1[((ds * N + t2) * M + a1) * V + s];
+#pragma unroll
+        for (int t1 = 0; t1 < N; ++t1) {
+            R acc = 0.;
+#pragma unroll
+            for (int a1 = 0; a1 < M; ++a1) acc = fma(tBw<R>(T, a1, t1), f[a1], acc);
+            l2[((ds * N + t2) * N + t1) * V + s] = acc;
+        }
+    });
+    sync();
+    // 6. expand along the normal directions into the residual: +F on the -
+    //    face (the element is the face's right one), -F on the + face
+    team_qs<NNN * V, TS>(dc, lane, [&](int b, int s, int t) {
+        const int i = b % N, j = (b / N) % N, k = b / NN;
+        R acc = sR[t];
+        acc += LT.phim[i] * l2[((0 * N + k) * N + j) * V + s] - LT.phip[i] * l2[((1 * N + k) * N + j) * V + s];
+        acc += LT.phim[j] * l2[((2 * N + k) * N + i) * V + s] - LT.phip[j] * l2[((3 * N + k) * N + i) * V + s];
+        acc += LT.phim[k] * l2[((4 * N + j) * N + i) * V + s] - LT.phip[k] * l2[((5 * N + j) * N + i) * V + s];
+        sR[t] = acc;
+    });
+    sync();
+}
+
+template <int P, int TS, int NR, class R>
+__device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& G, const TaskDec<TS, NR>& dc,
+                                                 int lane, R* sR, R* bankA, R* bankB) {
+    using Dm = Dims<P>;
+    constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, MMM = Dm::MMM;
+    const R* sU = bankA;
+    R* X1 = bankB;
+    R* X2 = bankA;
+    R* Uq = bankB;
+    R* Qd = bankA;
+    R* Cc = bankB;
+    R* Bv = bankA;
+
+    // x: X1[k][j][a][s] = sum_i B[a][i] U[k][j][i][s]
+    team_qs<NN * V, TS>(dc, lane, [&](int q, int s, int) {
+        R u[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) u[i] = sU[(q * N + i) * V + s];
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+            R acc = 0.;
+#pragma unroll
+            for (int i = 0; i < N; ++i) acc = fma(tB<R>(T, a, i), u[i], acc);
+            X1[(q * M + a) * V + s] = acc;
+        }
+    });
+    sync();
+    // y: X2[k][b][a][s] = sum_j B[b][j] X1[k][j][a][s]; line q = k M + a
+    // (consecutive lanes read consecutive LDS words)
+    team_qs<N * M * V, TS>(dc, lane, [&](int q, int s, int) {
+        const int a = q % M, k = q / M;
+        R u[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) u[j] = X1[((k * N + j) * M + a) * V + s];
+#pragma unroll
+        for (int b = 0; b < M; ++b) {
+            R acc = 0.;
+#pragma unroll
+            for (int j = 0; j < N; ++j) acc = fma(tB<R>(T, b, j), u[j], acc);
+            X2[((k * M + b) * M + a) * V + s] = acc;
+        }
+    });
+    sync();
+    // z: Uq[c][b][a][s] = sum_k B[c][k] X2[k][b][a][s]; line q = b M + a
+    team_qs<MM * V, TS>(dc, lane, [&](int q, int s, int) {
+        R u[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) u[k] = X2[(k * MM + q) * V + s];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            R acc = 0.;
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc = fma(tB<R>(T, c, k), u[k], acc);
+            Uq[(c * MM + q) * V + s] = acc;
+        }
+    });
+    sync();
+    // point fluxes: G_d[s] = F_d[s] (dxi_d / dx_d) |J| at each volume point,
+    // F_d[s] = U_s u_d + p delta_{s, d+1} (E + p in place of E)
+    team_for<MMM, TS>(lane, [&](int pq) {
+        const R* u = Uq + pq * V;
+        R U[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) U[v] = u[v];
+        const R r1 = frcp(U[0]);
+        const R p = pressure_r(R(G.gamma), U, r1);
+        U[4] += p;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const R g = R(G.gvol[d]);
+            const R ud = U[1 + d] * r1 * g, pd = p * g;
+            R* o = Qd + (d * MMM + pq) * V;
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = (v == 1 + d) ? fma(U[v], ud, pd) : U[v] * ud;
+        }
+    });
+    sync();
+    // c-step: contracted along z with the weighted factors: Bw for the x
+    // and y fluxes, Dw for the z flux; line q = b M + a
+    team_qs<MM * V, TS>(dc, lane, [&](int q, int s, int) {
+        R xx[N], yy[N], zz[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) xx[k] = yy[k] = zz[k] = 0.;
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const int o = (c * MM + q) * V + s;
+            const R gx = Qd[o], gy = Qd[MMM * V + o], gz = Qd[2 * MMM * V + o];
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                xx[k] = fma(tBw<R>(T, c, k), gx, xx[k]);
+                yy[k] = fma(tBw<R>(T, c, k), gy, yy[k]);
+                zz[k] = fma(tDw<R>(T, c, k), gz, zz[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            Cc[((0 * N + k) * MM + q) * V + s] = xx[k];
+            Cc[((1 * N + k) * MM + q) * V + s] = yy[k];
+            Cc[((2 * N + k) * MM + q) * V + s] = zz[k];
+        }
+    });
+    sync();
+    // b-step: along y (Bw for x and z fluxes, Dw for the y flux); line q = k M + a
+    team_qs<N * M * V, TS>(dc, lane, [&](int q, int s, int) {
+        const int a = q % M, k = q / M;
+        R cx[M], cy[M], cz[M];
+#pragma unroll
+        for (int b = 0; b < M; ++b) {
+            cx[b] = Cc[((0 * N + k) * MM + b * M + a) * V + s];
+            cy[b] = Cc[((1 * N + k) * MM + b * M + a) * V + s];
+            cz[b] = Cc[((2 * N + k) * MM + b * M + a) * V + s];
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            R y1 = 0., y2 = 0.;
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                y1 = fma(tBw<R>(T, b, j), cx[b], y1);
+                y2 = fma(tDw<R>(T, b, j), cy[b], y2);
+                y2 = fma(tBw<R>(T, b, j), cz[b], y2);
+            }
+            Bv[((0 * N + k) * N * M + j * M + a) * V + s] = y1;
+            Bv[((1 * N + k) * N * M + j * M + a) * V + s] = y2;
+        }
+    });
+    sync();
+    // a-step: along x (Dw for the x flux, Bw for the others), into sR
+    team_qs<NN * V, TS>(dc, lane, [&](int q, int s, int) {
+        R y1[M], y2[M];
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+            y1[a] = Bv[((0 * NN + q) * M + a) * V + s];
+            y2[a] = Bv[((1 * NN + q) * M + a) * V + s];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            R acc = 0.;
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                acc = fma(tDw<R>(T, a, i), y1[a], acc);
+                acc = fma(tBw<R>(T, a, i), y2[a], acc);
+            }
+            sR[(q * N + i) * V + s] += acc;
+        }
+    });
+    sync();
+}
+
 // Apply M^-1 (tensor product of 1D inverses, scaled by 1/|J|) in place on
 // an LDS coefficient block (without the dt and 1/|J| factors for the
 // non-diagonal case; the caller scales).
@@ -1260,7 +1657,7 @@ struct StageLoads {
     static constexpr int RN = (NT + TS - 1) / TS;
     double own[RU];
     double nl[RN][N];
-    double u0[RU], ac[RU];
+    double u0[RU], ac[RU], cm[RU];
     int nside[RN];          // general path: side of the neighbour's face
 
     __device__ __forceinline__ void load_block(const double* Ust, unsigned e, int lane) {
@@ -1294,9 +1691,7 @@ struct StageLoads {
             }
         }
     }
-    __device__ __forceinline__ void load_neighbours(const double* Ust, unsigned e, const KGeom& G, int lane) {
-        int ec[3];
-        elem_ijk(e, G, ec);
+    __device__ __forceinline__ void load_neighbours(const double* Ust, const int* ec, const KGeom& G, int lane) {
         const unsigned n0 = elem_at(ec[0] - 1, ec[1], ec[2], G), n1 = elem_at(ec[0] + 1, ec[1], ec[2], G);
         const unsigned n2 = elem_at(ec[0], ec[1] - 1, ec[2], G), n3 = elem_at(ec[0], ec[1] + 1, ec[2], G);
         const unsigned n4 = elem_at(ec[0], ec[1], ec[2] - 1, G), n5 = elem_at(ec[0], ec[1], ec[2] + 1, G);
@@ -1340,16 +1735,18 @@ struct StageLoads {
     __device__ __forceinline__ void load_rk(const StageArgs& A, unsigned e, int lane) {
         const size_t base = (size_t)e * NB;
         const bool need_acc = A.mode == MODE_RK_MID || A.mode == MODE_RK_LAST;
+        const bool need_cmp = A.cmp && (A.mode == MODE_RK_LAST || A.mode == MODE_FE);
 #pragma unroll
         for (int r = 0; r < RU; ++r) {
             const int t = r * TS + lane;
             u0[r] = t < NB ? A.U0[base + t] : 0.;
             ac[r] = (t < NB && need_acc) ? A.acc[base + t] : 0.;
+            cm[r] = (t < NB && need_cmp) ? A.cmp[base + t] : 0.;
         }
     }
     __device__ __forceinline__ void clear_rk() {
 #pragma unroll
-        for (int r = 0; r < RU; ++r) u0[r] = ac[r] = 0.;
+        for (int r = 0; r < RU; ++r) u0[r] = ac[r] = cm[r] = 0.;
     }
     // general path: the trace of the neighbour's face side nside
     __device__ __forceinline__ void store_neighbour_slabs_gen(const LaneTab& LT, double* slab, int lane) const {
@@ -1384,6 +1781,75 @@ struct StageLoads {
 #pragma unroll
                 for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
                 slab[((d * 4 + 2 + side) * NN + t2 * N + t1) * V + s] = acc;
+            }
+        }
+    }
+};
+
+// Neighbour lines on the box path (p >= 2), whole waves per neighbour: the
+// neighbour, its face direction and the line stride are uniform across a
+// wave, so the neighbour block's address is scalar, each lane adds one
+// 32-bit offset and the points along the line are immediate offsets.  The
+// trace that feeds the neighbour slab is the same fma chain as before.
+template <int P, int TS>
+struct BoxNbr {
+    using Dm = Dims<P>;
+    static constexpr int N = Dm::N, NN = Dm::NN, NB = Dm::NNN * V;
+    static constexpr int LN = NN * V;                      // line tasks per neighbour
+    static constexpr int WPN = (LN + WAVE - 1) / WAVE;     // waves per neighbour
+    static constexpr int WT = 6 * WPN;                     // waves of work
+    static constexpr int WR = TS / WAVE;                   // waves per round
+    static constexpr int RB = (WT + WR - 1) / WR;          // rounds
+    double nl[RB][N];
+
+    template <int DN>
+    __device__ __forceinline__ void line(const double* blk, int l, double* x) const {
+        const int s = l % V, u = l / V, t1 = u % N, t2 = u / N;
+        constexpr int sr = DN == 0 ? V : (DN == 1 ? N * V : NN * V);
+        constexpr int s1 = DN == 0 ? N * V : V;
+        constexpr int s2 = DN == 2 ? N * V : NN * V;
+        const double* ln = blk + (t1 * s1 + t2 * s2 + s);
+#pragma unroll
+        for (int q = 0; q < N; ++q) x[q] = ln[q * sr];
+    }
+    // nbe[ds]: neighbour element across face slot ds = d * 2 + side
+    __device__ __forceinline__ void load(const double* Ust, const unsigned* nbe, int lane) {
+        const int wv = __builtin_amdgcn_readfirstlane(lane / WAVE);
+        const int l0 = lane % WAVE;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int w = r * WR + wv;
+            if (w < WT) {
+                const int ds = w / WPN, l = (w - ds * WPN) * WAVE + l0;
+                if (l < LN) {
+                    const double* blk = Ust + (size_t)nbe[ds] * NB;
+                    const int dn = ds >> 1;
+                    if (dn == 0) line<0>(blk, l, nl[r]);
+                    else if (dn == 1) line<1>(blk, l, nl[r]);
+                    else line<2>(blk, l, nl[r]);
+                }
+            }
+        }
+    }
+    // the neighbour across the - face contributes its + trace and vice versa
+    template <class R>
+    __device__ __forceinline__ void store_slabs(const LaneTabT<R>& LT, R* slab, int lane) const {
+        const int wv = __builtin_amdgcn_readfirstlane(lane / WAVE);
+        const int l0 = lane % WAVE;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int w = r * WR + wv;
+            if (w < WT) {
+                const int ds = w / WPN, l = (w - ds * WPN) * WAVE + l0;
+                if (l < LN) {
+                    const int d = ds >> 1, side = ds & 1;
+                    const R* phi = side == 0 ? LT.phip : LT.phim;
+                    R acc = 0.;
+#pragma unroll
+                    for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
+                    // l = (t2 N + t1) V + s: the slab index of the line
+                    slab[(d * 4 + 2 + side) * LN + l] = acc;
+                }
             }
         }
     }
@@ -1440,24 +1906,28 @@ __device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int l
 // FP32 laid out like the state (the learned closure's corrected stage)
 template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>(), bool CORR = false>
 __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T, const KGeom& G, unsigned e,
-                                           const float* __restrict__ Corr = nullptr) {
+                                           const int* ec, const float* __restrict__ Corr = nullptr) {
     static_assert(sizeof(R) == 8 || !MAPPED, "the FP32 form is for the periodic box");
     static_assert(TS % WAVE == 0, "whole waves");
     using Dm = Dims<P>;
     using L = StageLDS<P, MAPPED>;
+    using LB = BoxLDS<P>;
     using PF = StageLoads<P, TS>;
     constexpr int NB = Dm::NNN * V;
-    static_assert(sizeof(R) == 8 || L::A * sizeof(R) >= NB * sizeof(double), "FP64 residual fits bank A");
-    __shared__ alignas(16) R lds[L::TOTAL];
+    constexpr int LA = MAPPED ? L::A : LB::A;
+    constexpr int LTOT = MAPPED ? L::TOTAL : LB::TOTAL;
+    static_assert(sizeof(R) == 8 || LA * sizeof(R) >= NB * sizeof(double), "FP64 residual fits bank A");
+    __shared__ alignas(16) R lds[LTOT];
     __shared__ LaneTabT<R> LT;
     // general path: node coordinates of the element and of its six face
     // neighbours (slot d*2 + side), for the left elements' normals
     __shared__ double XL[MAPPED ? 168 : 1];
     R* sR = lds;
     R* bankA = lds + L::R;
-    R* bankB = bankA + L::A;
+    R* bankB = bankA + LA;
     const int lane = threadIdx.x;
     const bool rk = A.mode != MODE_RESIDUAL;
+    const TaskDec<TS, box_rounds<P, TS>()> dc(lane);
 
     // lane tables first (their loads are the first to return), then every
     // global load of the element at once: its block, the neighbour lines
@@ -1479,11 +1949,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         });
     }
     PF pf;
+    BoxNbr<P, TS> bn;
+    constexpr bool WAVE_NBR = !MAPPED && P >= 2;
     pf.load_block(A.Ust, e, lane);
-    if constexpr (MAPPED) pf.load_neighbours_gen(A.Ust, e, G, lane);
-    else pf.load_neighbours(A.Ust, e, G, lane);
-    if (rk) pf.load_rk(A, e, lane);
-    else pf.clear_rk();
+    if constexpr (MAPPED) {
+        pf.load_neighbours_gen(A.Ust, e, G, lane);
+    } else if constexpr (WAVE_NBR) {
+        const unsigned nbe[6] = {elem_at(ec[0] - 1, ec[1], ec[2], G), elem_at(ec[0] + 1, ec[1], ec[2], G),
+                                 elem_at(ec[0], ec[1] - 1, ec[2], G), elem_at(ec[0], ec[1] + 1, ec[2], G),
+                                 elem_at(ec[0], ec[1], ec[2] - 1, G), elem_at(ec[0], ec[1], ec[2] + 1, G)};
+        bn.load(A.Ust, nbe, lane);
+    } else {
+        pf.load_neighbours(A.Ust, ec, G, lane);
+    }
     sync();
 #pragma unroll
     for (int r = 0; r < PF::RU; ++r) {
@@ -1494,10 +1972,16 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         }
     }
     if constexpr (MAPPED) pf.store_neighbour_slabs_gen(LT, bankB, lane);
+    else if constexpr (WAVE_NBR) bn.store_slabs(LT, bankB, lane);
     else pf.store_neighbour_slabs(LT, bankB, lane);
     sync();
-    face_phase<P, FLUX, TS, MAPPED, R>(T, LT, G, lane, bankA, sR, bankA, bankB, XL, e);
-    volume_phase<P, TS, MAPPED, R>(T, LT, G, lane, sR, bankA, bankB, XL);
+    if constexpr (MAPPED) {
+        face_phase<P, FLUX, TS, MAPPED, R>(T, LT, G, lane, bankA, sR, bankA, bankB, XL, e);
+        volume_phase<P, TS, MAPPED, R>(T, LT, G, lane, sR, bankA, bankB, XL);
+    } else {
+        face_phase_box<P, FLUX, TS>(T, LT, G, dc, lane, bankA, sR, bankA, bankB);
+        volume_phase_box<P, TS>(T, G, dc, lane, sR, bankA, bankB);
+    }
     // FP32 form: the residual widened to FP64 (bank A is free again)
     double* sRd;
     if constexpr (sizeof(R) == 8) {
@@ -1512,6 +1996,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         sync();
     }
 
+    // The RK registers are read only now: held from the start they would
+    // cost the whole stage their registers (and a wave per SIMD); other
+    // workgroups on the CU compute while these loads are in flight.
+    if (rk) pf.load_rk(A, e, lane);
+    else pf.clear_rk();
     // Epilogue with one store site per array; the value and the target are
     // picked by selects on the (uniform) mode.  Written as one branch per
     // mode, the backend has merged the stores into one block whose base
@@ -1526,6 +2015,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     const double scale = MAPPED ? A.dt : A.dt * G.jinv;
     double* const out = rk ? A.Uout : A.R;
     const bool wacc = mode == MODE_RK_FIRST || mode == MODE_RK_MID;
+    const bool last = mode == MODE_RK_LAST || mode == MODE_FE;
+    const bool wcmp = A.cmp && last;
 #pragma unroll
     for (int r = 0; r < PF::RU; ++r) {
         const int t = r * TS + lane;
@@ -1536,11 +2027,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
             if constexpr (CORR) dU += A.dt * (double)Corr[base + t];
             const double u0 = pf.u0[r], ac = pf.ac[r];
             const double n_rk = u0 + A.alpha * dU;                 // U0 + alpha dU
-            const double n_last = u0 + (1.0 / 6.0) * (ac + dU);    // U0 + (acc + dU) / 6
-            const double n_fe = u0 + dU;                           // U0 + dU
-            const double nv = !rk ? rv : (mode == MODE_RK_LAST ? n_last : (mode == MODE_FE ? n_fe : n_rk));
+            // the solution increment: (acc + dU) / 6 (RK4) or dU (FE), plus
+            // the carried rounding error of U0 when compensated
+            const double inc = (mode == MODE_RK_LAST ? (1.0 / 6.0) * (ac + dU) : dU) + pf.cm[r];
+            const double n_up = u0 + inc;
+            const double nv = !rk ? rv : (last ? n_up : n_rk);
             out[base + t] = nv;
             if (wacc) A.acc[base + t] = mode == MODE_RK_FIRST ? dU : ac + 2.0 * dU;
+            if (wcmp) {
+                // TwoSum: n_up + err = u0 + inc exactly
+                const double bb = n_up - u0;
+                A.cmp[base + t] = (u0 - (n_up - bb)) + (inc - bb);
+            }
         }
     }
 }
@@ -1548,14 +2046,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
 template <int P, int FLUX, bool MAPPED, class R = double>
 __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
-    stage_body<P, FLUX, MAPPED, R>(A, *Tp, G, block_element(G));
+    int ec[3];
+    const unsigned e = block_element_xyz(G, ec);
+    stage_body<P, FLUX, MAPPED, R>(A, *Tp, G, e, ec);
 }
 
 // The closure's corrected RK stage on an affine FP64 box (RK modes only)
 template <int P, int FLUX>
 __global__ __launch_bounds__(stage_team<P>()) void stage_corr_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                      KGeom G, const float* __restrict__ Corr) {
-    stage_body<P, FLUX, false, double, stage_team<P>(), true>(A, *Tp, G, block_element(G), Corr);
+    int ec[3];
+    const unsigned e = block_element_xyz(G, ec);
+    stage_body<P, FLUX, false, double, stage_team<P>(), true>(A, *Tp, G, e, ec, Corr);
 }
 
 // Batched stage: the elements of many independent box cases of one order
@@ -1585,12 +2087,15 @@ __global__ __launch_bounds__(TS) void stage_batch_kernel(const BatchCase* __rest
     A.U0 = reinterpret_cast<const double*>(base + ou0);
     A.Uout = reinterpret_cast<double*>(const_cast<char*>(base) + oout);
     A.acc = reinterpret_cast<double*>(const_cast<char*>(base) + C->off_acc);
+    A.cmp = C->off_cmp >= 0 ? reinterpret_cast<double*>(const_cast<char*>(base) + C->off_cmp) : nullptr;
     A.R = nullptr;
     A.dt = C->dt;
     A.alpha = stage == 2 ? 1.0 : 0.5;
     A.mode = stage == 0 ? MODE_RK_FIRST : (stage == 3 ? MODE_RK_LAST : MODE_RK_MID);
     const KGeom G = C->g;
-    stage_body<P, FLUX, false, R, TS>(A, *Tp, G, brick_element(G, q - C->first_block));
+    int ec[3];
+    const unsigned e = brick_element_xyz(G, q - C->first_block, ec);
+    stage_body<P, FLUX, false, R, TS>(A, *Tp, G, e, ec);
 }
 
 // dU = dt M^-1 R

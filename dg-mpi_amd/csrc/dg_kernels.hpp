@@ -32,6 +32,14 @@ struct KTables {
     float phim32[PMAX + 1];
     float phip32[PMAX + 1];
     float w32[PMAX + 2];
+    // the 1D factors with the (p+2)-point weight folded in (w_a B[a][i],
+    // w_a D[a][i]): the weak-gradient projection and the face lift of the
+    // box stage contract with these, so no weight products are formed per
+    // quadrature point
+    double Bw[PMAX + 2][PMAX + 1];
+    double Dw[PMAX + 2][PMAX + 1];
+    float Bw32[PMAX + 2][PMAX + 1];
+    float Dw32[PMAX + 2][PMAX + 1];
 };
 
 // Structured periodic box and physics constants.  dim = 2: nx x ny box of
@@ -73,6 +81,10 @@ enum StageMode {
     MODE_RK_LAST = 3,    // U0 += (acc + dU) / 6                    (RK4 stage 4)
     MODE_FE = 4          // Uout = U0 + dU                          (forward Euler)
 };
+// With a compensation register (StageArgs::cmp) the last RK4 stage and
+// forward Euler form Uout = U0 + (inc + cmp) by TwoSum and keep its
+// rounding error in cmp, so U0 + cmp carries the solution to about twice
+// the FP64 precision over many small steps.
 
 struct StageArgs {
     const double* Ust;   // state the residual is evaluated at (neighbours read it)
@@ -80,6 +92,7 @@ struct StageArgs {
     double* Uout;        // next stage state
     double* acc;         // RK accumulator
     double* R;           // residual output (MODE_RESIDUAL)
+    double* cmp;         // compensated update: rounding error of U0 (null: plain)
     double dt;           // dU = dt M^-1 R
     double alpha;        // stage coefficient
     int mode;
@@ -93,6 +106,7 @@ struct StageArgs {
 // first_block + E - 1).
 struct BatchCase {
     long long off_U0, off_Ua, off_Ub, off_acc;
+    long long off_cmp;       // compensation register, -1: plain update
     double dt;
     KGeom g;
     unsigned first_block;

@@ -112,6 +112,7 @@ void step_once(dg_solver* s, double dt) {
     using namespace dg;
     StageArgs a{};
     a.dt = dt;
+    a.cmp = s->cmp;
     hipStream_t st = s->stream;
     const int flux = (int)s->cfg.flux;
     if (s->cfg.stepper == Stepper::FE) {
@@ -139,11 +140,17 @@ void step_once(dg_solver* s, double dt) {
     hip_check(launch_stage(s->P, flux, a, s->dtab, s->geo, st), "stage kernel");
 }
 
+// a new state starts without carried rounding error
+void clear_cmp(dg_solver* s) {
+    if (s->cmp) hip_check(hipMemsetAsync(s->cmp, 0, s->ncoef * sizeof(double), s->stream), "hipMemset(cmp)");
+}
+
 void free_all(dg_solver* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
-    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work, s->X, s->imm, s->nev})
+    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->cmp, s->work, s->X, s->imm, s->nev})
         if (p) (void)hipFree(p);
+    s->cmp = nullptr;
     if (s->frec) (void)hipFree(s->frec);
     s->frec = nullptr;
     s->nev = nullptr;
@@ -243,6 +250,10 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         hip_check(hipMalloc(&s->dtab, sizeof(dg::KTables)), "hipMalloc(tables)");
         hip_check(hipMemcpy(s->dtab, &s->tab, sizeof(dg::KTables), hipMemcpyHostToDevice), "hipMemcpy(tables)");
         hip_check(hipMemsetAsync(s->U0, 0, bytes, s->stream), "hipMemset");
+        if (cfg.compensated) {
+            hip_check(hipMalloc(&s->cmp, bytes), "hipMalloc(cmp)");
+            hip_check(hipMemsetAsync(s->cmp, 0, bytes, s->stream), "hipMemset");
+        }
         if (s->geo.mapped) {
             // upload the general-path tables, then form the per-element
             // inverse mass matrices (precompute_matrix_helpers,
@@ -397,6 +408,7 @@ int dg_init_state(dg_solver* s) {
         set_device(s);
         s->time = 0.0;
         hip_check(dg::launch_init_state(s->P, s->time, s->U0, s->dtab, s->geo, s->ic, s->stream), "init kernel");
+        clear_cmp(s);
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
         if (s->cfg.timestep <= 0.0) {
             // CFL rule (new): dt = cfl * min(h) / max(|u| + c) of the initial state
@@ -429,6 +441,8 @@ int dg_set_state(dg_solver* s, const double* host, int layout) {
         } else {
             throw std::invalid_argument("unknown layout");
         }
+        clear_cmp(s);
+        hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     });
 }
 
@@ -632,6 +646,7 @@ void upload_group(dg_batch* b, dg_batch::Group& g) {
         c.off_Ua = off(s->Ua);
         c.off_Ub = off(s->Ub);
         c.off_acc = off(s->acc);
+        c.off_cmp = s->cmp ? off(s->cmp) : -1;
         c.dt = s->dt;
         c.g = s->geo;
         c.first_block = first;

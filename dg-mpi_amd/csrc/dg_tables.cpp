@@ -174,6 +174,10 @@ KTables make_tables(const CaseConfig& c) {
         for (int i = 0; i < PMAX + 1; ++i) {
             t.B32[a][i] = (float)t.B[a][i];
             t.D32[a][i] = (float)t.D[a][i];
+            t.Bw[a][i] = t.w[a] * t.B[a][i];
+            t.Dw[a][i] = t.w[a] * t.D[a][i];
+            t.Bw32[a][i] = (float)t.Bw[a][i];
+            t.Dw32[a][i] = (float)t.Dw[a][i];
         }
         t.w32[a] = (float)t.w[a];
     }

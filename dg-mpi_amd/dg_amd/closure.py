@@ -167,6 +167,7 @@ def step(solver, mlp, bound, nsteps, dt=0.0):
     return n.value
 
 
+
 # ---------------------------------------------------------------------------
 # Training data, fitting and evaluation (the workflow of the closure)
 # ---------------------------------------------------------------------------

@@ -5,16 +5,19 @@
 * the z-extruded isentropic vortex (rho=1, u=v=1, w=0, p=1, beta=5);
 * a Gaussian density bump (amplitude 0.2, width 1.0) carried by a uniform
   flow of speed 1 in a seeded random direction (numpy default_rng(2024):
-  a normalised Gaussian vector), p = 1.
+  a normalised Gaussian vector), p = 1, at CFL 0.06: at this Mach number
+  (0.85) RK4 with p = 3 is unstable at CFL 0.1 (the oracle's own run on a
+  6^3 box blows up within 50 steps at 0.1 and stays bounded at 0.06).
 
 Checked: finite states; mass, momentum and energy conserved (FP64: 1e-12 of
 the largest integral; FP32: 1e-6); the L2 error against the exact advected
-solution below stated bounds (derived from the IC projection error at t=0,
-which the 50 steps may at most double, plus the measured growth); the FP32
-form's L2 error within 2x the FP64 form's for rho, rho u, rho v, rho E (at
-this resolution the discretisation error, ~1e-7 in rho, is of the size of
-FP32 rounding of O(1) states, so the FP32 form cannot stay within 1.05x);
-the ratio is printed.  One RK4 step at 32^3 of the same case against the
+solution below a stated bound (three times the IC projection error at
+t=0: 50 steps, t ~ 0.2, of a resolved smooth solution); the FP32
+form's L2 error within 2x the FP64 form's, or within 1e-6 where the FP64
+error is below FP32 rounding (at this resolution the discretisation error,
+~1e-7 in rho for the vortex and ~3e-8 for the bump, is of the size of FP32
+rounding of O(1) states, so the FP32 form cannot stay within 1.05x); the
+ratio is printed.  One RK4 step at 32^3 of the same case against the
 FP64 oracle is in test_gpu_headline_oracle_step (bar 1e-11 for FP64,
 2e-6 for FP32, relative to the largest state value)."""
 import numpy as np
@@ -36,8 +39,9 @@ def bump_data(seed=2024):
 
 
 def toml(ic, prec, ne=NE):
+    cfl = 0.06 if ic[0] == "GaussianBump" else 0.1
     return dg_amd.case_toml(order=P, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne), lower=(-5., -5., -5.),
-                            upper=(5., 5., 5.), stepper="RK4", cfl=0.1, end=1.0, ic=ic[0], ic_data=ic[1],
+                            upper=(5., 5., 5.), stepper="RK4", cfl=cfl, end=1.0, ic=ic[0], ic_data=ic[1],
                             precision=prec)
 
 
@@ -72,7 +76,10 @@ def test_headline_64cube_p3_roe_50_steps(ic):
     # the projection error at t = 0 bounds the scale: 50 steps (t ~ 0.23) of
     # a resolved smooth solution may at most triple it
     assert np.all(r64["e"][v] < 3.0 * r64["e0"][v] + 1e-12), (r64["e"], r64["e0"])
-    assert np.all(ratio < 2.0), ratio
+    # FP32 form: within 2x the FP64 error, or within 1e-6 (FP32 rounding of
+    # O(1) states through 50 steps) where the discretisation error is below
+    # that (the bump: ~3e-8)
+    assert np.all(r32["e"][v] < np.maximum(2.0 * r64["e"][v], 1e-6)), ratio
     assert diff < 1e-5
 
 
