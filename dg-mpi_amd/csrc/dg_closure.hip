@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -462,6 +464,20 @@ __global__ void target_at_kernel(const double* __restrict__ Df, const double* __
     Y[t] = (float)(Df[f] - Dc[t] + (Uf[f] - Uc[t]) * rtau);
 }
 
+// Device scratch of dg_closure_step_noalloc: the feature rows [E][NF], the
+// corrections [E][NO] and the 64-bit fallback counter of up to
+// NOALLOC_MAX_ELEMS elements (32^3, the closure's box), 14.4 MB, in a
+// file-scope array that each GPU holds until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): it uses the address hipGetSymbolAddress returns
+// for the current device (noalloc_scratch).
+constexpr long NOALLOC_MAX_ELEMS = 32768;
+constexpr size_t NOALLOC_X = (size_t)NOALLOC_MAX_ELEMS * NF;        // floats
+constexpr size_t NOALLOC_C = (size_t)NOALLOC_MAX_ELEMS * NO;        // floats
+constexpr size_t NOALLOC_FLOATS = NOALLOC_X + NOALLOC_C + 2;        // + the counter (8 B, 8-aligned)
+static_assert(((NOALLOC_X + NOALLOC_C) * sizeof(float)) % 8 == 0, "counter alignment");
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
 // one thread clears *p (a vector store: the address is lane-indexed)
 __global__ void clear_u64_kernel(unsigned long long* p) { p[threadIdx.x] = 0ull; }
 
@@ -622,6 +638,23 @@ void tendency(dg_solver* s, const double* U, double* D) {
     a.mode = dg::MODE_RESIDUAL;
     hcheck(dg::launch_stage(s->P, (int)s->cfg.flux, a, s->dtab, s->geo, s->stream), "stage kernel");
     hcheck(dg::launch_inv_mass(s->P, 1.0, s->Ua, D, s->dtab, s->geo, s->stream), "inverse mass kernel");
+}
+
+// The device address of `scratch` on the current device, looked up once
+// per device (under a lock) and kept; no heap memory is involved.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+float* noalloc_scratch(int device) {
+    static std::mutex mu;
+    static float* addr[NOALLOC_MAX_DEVICES] = {};
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw std::invalid_argument("device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void* ptr = nullptr;
+        hcheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned NULL");
+        addr[device] = static_cast<float*>(ptr);
+    }
+    return addr[device];
 }
 
 // argument checks of the closure step; returns the time step
@@ -934,6 +967,22 @@ int dg_closure_step(dg_solver* s, dg_mlp* m, double dt_step, double bound, int n
         if (!s->cl_C) hcheck(hipMalloc(&s->cl_C, (size_t)E * NO * sizeof(float)), "hipMalloc");
         if (!s->cl_cnt) hcheck(hipMalloc(&s->cl_cnt, sizeof(unsigned long long)), "hipMalloc");
         closure_steps(s, m, dt, bound, nsteps, s->cl_X, s->cl_C, s->cl_cnt, n_fallback);
+    });
+}
+
+int dg_closure_step_noalloc(dg_solver* s, dg_mlp* m, double dt_step, double bound, int nsteps, long* n_fallback) {
+    return mlp_guard(g_mlp_error, [&] {
+        const double dt = closure_check(s, m, dt_step, nsteps);
+        const long E = s->geo.E;
+        if (E > NOALLOC_MAX_ELEMS)
+            throw std::invalid_argument("dg_closure_step_noalloc: " + std::to_string(E) + " elements exceed the " +
+                                        std::to_string(NOALLOC_MAX_ELEMS) + " of its static scratch (use dg_closure_step)");
+        // the device is current (closure_check): its own copy of the array
+        float* base = noalloc_scratch(s->device);
+        float* X = base;
+        float* C = base + NOALLOC_X;
+        auto* cnt = reinterpret_cast<unsigned long long*>(base + NOALLOC_X + NOALLOC_C);
+        closure_steps(s, m, dt, bound, nsteps, X, C, cnt, n_fallback);
     });
 }
 

@@ -39,7 +39,7 @@ EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get
            "dg_batch_destroy", "dg_batch_last_error", "dg_batch_tuning", "dg_mlp_create", "dg_mlp_destroy", "dg_mlp_last_error",
            "dg_mlp_num_params", "dg_mlp_get_params", "dg_mlp_set_params", "dg_mlp_set_scaling", "dg_mlp_forward",
            "dg_mlp_gradient", "dg_mlp_train", "dg_closure_features", "dg_closure_target", "dg_closure_target_at",
-           "dg_closure_step"]
+           "dg_closure_step", "dg_closure_step_noalloc"]
 
 
 class DGError(RuntimeError):
@@ -107,6 +107,7 @@ def lib():
             "dg_closure_target": [vp, vp, vp, vp, vp],
             "dg_closure_target_at": [vp, vp, vp, vp, C.c_double, vp],
             "dg_closure_step": [vp, vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_long)],
+            "dg_closure_step_noalloc": [vp, vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_long)],
         }
         for name, args in sig.items():
             f = getattr(L, name)

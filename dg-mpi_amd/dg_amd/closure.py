@@ -167,6 +167,14 @@ def step(solver, mlp, bound, nsteps, dt=0.0):
     return n.value
 
 
+def step_noalloc(solver, mlp, bound, nsteps, dt=0.0):
+    """step() through dg_closure_step_noalloc (the library's static device
+    scratch, at most 32^3 elements; no device allocation)."""
+    n = C.c_long()
+    _check(lib().dg_closure_step_noalloc(solver._h, mlp._h, dt, bound, nsteps, C.byref(n)))
+    solver.info = solver._info()
+    return n.value
+
 
 # ---------------------------------------------------------------------------
 # Training data, fitting and evaluation (the workflow of the closure)

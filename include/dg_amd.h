@@ -262,6 +262,18 @@ int dg_closure_target_at(dg_solver* fine, const double* Uf_dev, dg_solver* coars
  * 2-norm above `bound` get none (the plain residual), counted over all
  * stages in *n_fallback */
 int dg_closure_step(dg_solver* s, dg_mlp* m, double dt, double bound, int nsteps, long* n_fallback);
+/* dg_closure_step without any memory allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The feature rows, corrections and fallback counter live in
+ * a static device array of the library (14.4 MB, held by each GPU the
+ * library runs on until the process ends) instead of buffers on the
+ * handle; it takes at most 32768 elements (32^3) and returns
+ * DG_ERR_ARGUMENT above that.  Every call on one GPU shares that GPU's
+ * array, which keeps the last call's contents: calls that overlap on one
+ * GPU (several host threads, several streams) race.  Results equal
+ * dg_closure_step's. */
+int dg_closure_step_noalloc(dg_solver* s, dg_mlp* m, double dt, double bound, int nsteps, long* n_fallback);
 
 #ifdef __cplusplus
 }
