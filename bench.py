@@ -14,10 +14,12 @@ value = DOF-updates per second per RK stage = E (p+1)^3 / t_stage.
 
 Accuracy: after the timed steps the run continues to 50 RK4 steps in all
 (the configuration's count) and reports the L2 error against the exact
-advected vortex.  Beside it, "fp32_form": the same run with
+advected vortex.  Beside it, "fp32_form", "bf16x3_form", "bf16_form": the same run with
 Numerics.precision = "FP32" (face and volume work of the stage kernel in
-FP32, RK registers, M^-1 and the L2 reductions in FP64): its rate, and its
-L2 error against the FP64 run's; "gaussian_bump": the configuration's second
+FP32, RK registers, M^-1 and the L2 reductions in FP64, compensated
+update), "BF16x3" and "BF16" (the FP32 form with the x-y projection as a
+bf16 matrix-core GEMM, hi/lo split or one term): rates, and L2 errors
+against the FP64 run's; "gaussian_bump": the configuration's second
 input (a density bump in uniform flow along a seeded direction), both
 forms, 50 steps; "l2_err_vs_fp64_oracle": one RK4 step of both forms on a
 32^3 box of the same case against the FP64 CPU oracle from the same state
@@ -494,7 +496,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     # the FP32 form only (for profiling its kernel); --no-fp32 skips it
     ap.add_argument("--fp32", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the reduced-precision legs")
+    # one form only (for profiling its kernel): FP32, BF16 or BF16x3
+    ap.add_argument("--precision", default=None, help=argparse.SUPPRESS)
     # batched uneven cases (dg_batch_*): number of cases and their seed
     ap.add_argument("--batch", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--seed", type=int, default=7, help=argparse.SUPPRESS)
@@ -559,7 +563,7 @@ def main():
         s.init_state_from_fcn()
         io_bench(s, args.io)
         return
-    s, ms_per_step, wall = timed_run("FP32" if args.fp32 else None)
+    s, ms_per_step, wall = timed_run(args.precision or ("FP32" if args.fp32 else None))
     dt = s.timestep
     t_stage = ms_per_step / 4 * 1e-3
     value = dof / t_stage
@@ -591,7 +595,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.fp32 else "f64",
+        "dtype": (args.precision or ("FP32" if args.fp32 else "f64")).lower(),
         "data": "synthetic (z-extruded isentropic vortex, generated on the GPU)" if dim == 3 else
                 "synthetic (2D isentropic vortex, generated on the GPU)",
         "config": {"workload": (f"euler3d_periodic_box_{ne}^3_p{order}_legendre_roe_rk4_cfl0.1"
@@ -617,31 +621,39 @@ def main():
         "wall_s": wall,
     }
     gpu_step = None
-    if dim == 3 and not args.warp and not args.fp32 and not args.no_fp32:
-        # the FP32 form on the same workload, against this FP64 run
+    if dim == 3 and not args.warp and not args.fp32 and not args.precision and not args.no_fp32:
+        # the reduced-precision forms on the same workload, against this FP64
+        # run: FP32 (contractions and fluxes on the VALU), BF16 and BF16x3
+        # (the FP32 form with the x-y projection on bf16 matrix cores: one
+        # bf16 term / a hi-lo split with three products); all keep FP64 RK
+        # registers with the compensated update, M^-1 and reductions
         U64 = s.get_state()
         s.close()
-        s32, ms32, _ = timed_run("FP32")
-        to_acc_steps(s32)
-        err32 = s32.get_error()
-        U32 = s32.get_state()
-        diff = float(np.max(np.abs(U32 - U64)) / np.max(np.abs(U64)))
-        del U32, U64
-        out["fp32_form"] = {
-            "value": dof / (ms32 / 4 * 1e-3), "unit": "DOF-updates/s", "ms_per_step": ms32,
-            "speedup_vs_fp64": ms_per_step / ms32,
-            "dtype": "f32 traces/fluxes/contractions, f64 RK registers, M^-1 and reductions",
-            "kernel": f"stage_kernel<{order},Roe,fp32>",
-            "l2_error_vs_exact": [float(x) for x in err32],
-            # the extruded vortex has w = 0 exactly: FP64 keeps it at round-off (~1e-16) and
-            # FP32 at ~1e-8, so the ratio is reported for the variables with a real error
-            "l2_error_ratio_vs_fp64": [float(a / b) if b > 1e-12 else None for a, b in zip(err32, err)],
-            "state_max_rel_diff_vs_fp64": diff,
-        }
-        s32.close()
+        for key, prec, what in (("fp32_form", "FP32", "f32 traces/fluxes/contractions"),
+                                ("bf16x3_form", "BF16x3", "f32, x-y projection as bf16 hi/lo MFMA (3 products)"),
+                                ("bf16_form", "BF16", "f32, x-y projection as bf16 MFMA")):
+            sr, msr, _ = timed_run(prec)
+            to_acc_steps(sr)
+            errr = sr.get_error()
+            Ur = sr.get_state()
+            diff = float(np.max(np.abs(Ur - U64)) / np.max(np.abs(U64)))
+            del Ur
+            out[key] = {
+                "value": dof / (msr / 4 * 1e-3), "unit": "DOF-updates/s", "ms_per_step": msr,
+                "speedup_vs_fp64": ms_per_step / msr,
+                "dtype": what + ", f64 RK registers (compensated update), M^-1 and reductions",
+                "kernel": f"stage_kernel<{order},Roe,fp32" + ("" if prec == "FP32" else f",{prec.lower()}") + ">",
+                "l2_error_vs_exact": [float(x) for x in errr],
+                # the extruded vortex has w = 0 exactly: FP64 keeps it at round-off (~1e-16) and
+                # FP32 at ~1e-8, so the ratio is reported for the variables with a real error
+                "l2_error_ratio_vs_fp64": [float(a / b) if b > 1e-12 else None for a, b in zip(errr, err)],
+                "state_max_rel_diff_vs_fp64": diff,
+            }
+            sr.close()
+        del U64
     else:
         s.close()
-    if headline and not args.fp32 and not args.no_fp32:
+    if headline and not args.fp32 and not args.precision and not args.no_fp32:
         # the second input of the configuration: a Gaussian density bump
         # (amplitude 0.2, width 1) in uniform flow of speed 1 along a seeded
         # random direction (numpy default_rng(2024), a normalised Gaussian

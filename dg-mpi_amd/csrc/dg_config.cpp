@@ -273,7 +273,13 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     int qi = pick(quad, {"GaussLegendre", "GaussLobatto", "TriangleDunavant", "TriangleGaussLegendre"});
     if (qi != 0) throw UnsupportedError("Numerics.quad = " + quad + " is not implemented");
     if (c.order < 1) throw UnsupportedError("Numerics.order must be >= 1 on this path");
-    c.fp32 = pick(get_str(d, "Numerics", "precision", false, "FP64"), {"FP64", "FP32"}) == 1;
+    // FP32: the FP32 form; BF16 / BF16x3: the FP32 form with the x-y part of
+    // the weak-gradient projection on the bf16 matrix cores (FP32
+    // accumulation; one bf16 term, or the value and its rounding error as
+    // two bf16 terms and three products)
+    const int prec = pick(get_str(d, "Numerics", "precision", false, "FP64"), {"FP64", "FP32", "BF16", "BF16x3"});
+    c.fp32 = prec >= 1;
+    c.proj = prec == 2 ? 1 : (prec == 3 ? 3 : 0);
     if (c.fp32 && dim != 3) throw UnsupportedError("Numerics.precision = FP32 with Physics.dim = 2 (3D box only)");
     if (c.basis == Basis::LegendreHex && c.order > 7)
         throw InputError("Legendre polynomial P = " + std::to_string(c.order) + " not implemented.");

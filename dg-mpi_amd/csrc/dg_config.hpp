@@ -54,6 +54,9 @@ struct CaseConfig {
     // reference counterpart): FP32 face and volume work in the stage
     // kernel, FP64 RK registers, M^-1 and reductions; 3D periodic box only
     bool fp32 = false;
+    // "BF16" (1) / "BF16x3" (3): the FP32 form with the x-y projection on
+    // bf16 matrix cores (one term / a hi-lo split with three products)
+    int proj = 0;
     // [Mesh]: structured periodic box, type = "PeriodicBox", nelem, lower,
     // upper; warp > 0 displaces the nodes periodically (trilinear,
     // non-affine elements); normals = "reference" (the reference's

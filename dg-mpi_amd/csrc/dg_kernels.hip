@@ -1238,7 +1238,20 @@ __device__ __forceinline__ void volume_phase(const KTables& T, const LaneTabT<R>
 //   * The Riemann step picks the left / right trace by address, not by
 //     per-value selects.
 // ---------------------------------------------------------------------------
+// The bf16 matrix-core form of the x-y projection: out[m][n] =
+// sum_kk Kproj[m][kk] G[kk][n], m = j (p+1) + i, kk = dir (p+2)^2 + b (p+2) + a,
+// n = k V + s, in 16 x 16 x 16 tiles (MT x NT output tiles, KC k-steps); G
+// sits in LDS as [n][kk] bf16 rows of KP entries (padded off the bank period).
 template <int P>
+struct ProjDims {
+    static constexpr int N = P + 1, M = P + 2, NN = N * N, MM = M * M;
+    static constexpr int MT = (NN + 15) / 16, NT = (N * V + 15) / 16, KC = (3 * MM + 15) / 16;
+    static constexpr int KP = KC * 16 + 4;
+    static constexpr int GROWS = NT * 16;
+    static constexpr int G16 = GROWS * KP;          // bf16 entries of one G part
+};
+
+template <int P, int GBF = 0>
 struct BoxLDS {
     using Dm = Dims<P>;
     static constexpr int N = Dm::N, M = Dm::M;
@@ -1256,11 +1269,71 @@ struct BoxLDS {
     static constexpr int C = 3 * N * M * M * V;
     static constexpr int BV = 2 * N * N * M * V;
     static constexpr int A = cmax(cmax(U + cmax(cmax(F1, FH), L2), X2), cmax(QD, BV));
-    static constexpr int Bk = cmax(cmax(cmax(SLAB, F2), L1), cmax(cmax(X1, UQ), C));
+    static constexpr int Bk = cmax(cmax(cmax(cmax(SLAB, F2), L1), cmax(cmax(X1, UQ), C)), GBF);
     static constexpr int R = Dm::NNN * V;
     static constexpr int TOTAL = R + A + Bk;
     static constexpr int RP = ((2 * Dm::MM + WAVE - 1) / WAVE) * WAVE;
 };
+
+// bank-B space (in units of R) that the bf16 projection's G operand takes
+template <int P, int PROJ, class R>
+constexpr int proj_bank() {
+    return PROJ == 0 ? 0 : (ProjDims<P>::G16 * (PROJ == 3 ? 2 : 1) * 2 + (int)sizeof(R) - 1) / (int)sizeof(R);
+}
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// bf16 of a float, round to nearest even, and back
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// x-y projection on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation), added into the FP32 residual sR.  Operand layout of the
+// 16x16x16 bf16 MFMA (lane l, element q = 0..3): A[l % 16][4 (l / 16) + q],
+// B[4 (l / 16) + q][l % 16]; lane l receives D[4 (l / 16) + q][l % 16].
+// Each wave takes whole output tiles; PROJ = 3 adds the products of the
+// hi part with the lo part and of the lo part with the hi part.
+template <int P, int TS, int PROJ>
+__device__ __forceinline__ void proj_mfma(const KTables& T, int lane, const uint16_t* Gb, float* sR) {
+    using PD = ProjDims<P>;
+    constexpr int NW = TS / WAVE;
+    const int wv = __builtin_amdgcn_readfirstlane(lane / WAVE);
+    const int l = lane % WAVE, c = l % 16, kq = 4 * (l / 16);
+    for (int tile = wv; tile < PD::MT * PD::NT; tile += NW) {
+        const int mt = tile / PD::NT, nt = tile % PD::NT;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        const uint16_t* arow0 = &T.Kproj[0][mt * 16 + c][kq];
+        const uint16_t* arow1 = &T.Kproj[1][mt * 16 + c][kq];
+        const uint16_t* brow0 = Gb + (nt * 16 + c) * PD::KP + kq;
+        const uint16_t* brow1 = brow0 + PD::G16;
+#pragma unroll
+        for (int kc = 0; kc < PD::KC; ++kc) {
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(arow0 + 16 * kc);
+            const bf16x4 b = *reinterpret_cast<const bf16x4*>(brow0 + 16 * kc);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            if constexpr (PROJ == 3) {
+                const bf16x4 al = *reinterpret_cast<const bf16x4*>(arow1 + 16 * kc);
+                const bf16x4 bl = *reinterpret_cast<const bf16x4*>(brow1 + 16 * kc);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b, acc, 0, 0, 0);
+            }
+        }
+        const int n = nt * 16 + c;
+        if (n < PD::N * V) {
+            const int k = n / V, sv = n - V * k;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = mt * 16 + kq + q;
+                if (m < PD::NN) sR[(k * PD::NN + m) * V + sv] += acc[q];
+            }
+        }
+    }
+}
 
 // per-lane task decode of up to NR rounds of a team of TS threads
 template <int TS, int NR>
@@ -1421,9 +1494,10 @@ __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<
     sync();
 }
 
-template <int P, int TS, int NR, class R>
+template <int P, int TS, int NR, class R, int PROJ = 0>
 __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& G, const TaskDec<TS, NR>& dc,
                                                  int lane, R* sR, R* bankA, R* bankB) {
+    static_assert(PROJ == 0 || sizeof(R) == 4, "the bf16 projection belongs to the FP32 form");
     using Dm = Dims<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, MMM = Dm::MMM;
     const R* sU = bankA;
@@ -1515,14 +1589,50 @@ __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& 
                 zz[k] = fma(tDw<R>(T, c, k), gz, zz[k]);
             }
         }
+        if constexpr (PROJ == 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            Cc[((0 * N + k) * MM + q) * V + s] = xx[k];
-            Cc[((1 * N + k) * MM + q) * V + s] = yy[k];
-            Cc[((2 * N + k) * MM + q) * V + s] = zz[k];
+            for (int k = 0; k < N; ++k) {
+                Cc[((0 * N + k) * MM + q) * V + s] = xx[k];
+                Cc[((1 * N + k) * MM + q) * V + s] = yy[k];
+                Cc[((2 * N + k) * MM + q) * V + s] = zz[k];
+            }
+        } else {
+            // the matrix cores' G operand, [n = k V + s][kk = dir MM + q], bf16
+            using PD = ProjDims<P>;
+            uint16_t* Gb = reinterpret_cast<uint16_t*>(bankB);
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                uint16_t* row = Gb + (k * V + s) * PD::KP + q;
+                const float g[3] = {float(xx[k]), float(yy[k]), float(zz[k])};
+#pragma unroll
+                for (int dir = 0; dir < 3; ++dir) {
+                    const uint16_t hi = bf16_rne(g[dir]);
+                    row[dir * MM] = hi;
+                    if constexpr (PROJ == 3) row[PD::G16 + dir * MM] = bf16_rne(g[dir] - bf16_value(hi));
+                }
+            }
         }
     });
+    if constexpr (PROJ != 0) {
+        // zero k-padding of the G rows (columns 3 MM .. 16 KC), so that the
+        // padded k-steps add 0 x 0
+        using PD = ProjDims<P>;
+        constexpr int PADC = PD::KC * 16 - 3 * MM;
+        if constexpr (PADC > 0) {
+            uint16_t* Gb = reinterpret_cast<uint16_t*>(bankB);
+            team_for<N * V * PADC, TS>(lane, [&](int t) {
+                const int n = t / PADC, kk = 3 * MM + t % PADC;
+                Gb[n * PD::KP + kk] = 0;
+                if constexpr (PROJ == 3) Gb[PD::G16 + n * PD::KP + kk] = 0;
+            });
+        }
+    }
     sync();
+    if constexpr (PROJ != 0) {
+        proj_mfma<P, TS, PROJ>(T, lane, reinterpret_cast<const uint16_t*>(bankB), sR);
+        sync();
+        return;
+    }
     // b-step: along y (Bw for x and z fluxes, Dw for the y flux); line q = k M + a
     team_qs<N * M * V, TS>(dc, lane, [&](int q, int s, int) {
         const int a = q % M, k = q / M;
@@ -1904,14 +2014,14 @@ __device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int l
 // which read and write the FP64 registers.  Periodic box only.
 // CORR: the RK update takes dU = dt (M^-1 R + C), C a per-DOF correction in
 // FP32 laid out like the state (the learned closure's corrected stage)
-template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>(), bool CORR = false>
+template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>(), bool CORR = false, int PROJ = 0>
 __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T, const KGeom& G, unsigned e,
                                            const int* ec, const float* __restrict__ Corr = nullptr) {
     static_assert(sizeof(R) == 8 || !MAPPED, "the FP32 form is for the periodic box");
     static_assert(TS % WAVE == 0, "whole waves");
     using Dm = Dims<P>;
     using L = StageLDS<P, MAPPED>;
-    using LB = BoxLDS<P>;
+    using LB = BoxLDS<P, proj_bank<P, PROJ, R>()>;
     using PF = StageLoads<P, TS>;
     constexpr int NB = Dm::NNN * V;
     constexpr int LA = MAPPED ? L::A : LB::A;
@@ -1980,7 +2090,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         volume_phase<P, TS, MAPPED, R>(T, LT, G, lane, sR, bankA, bankB, XL);
     } else {
         face_phase_box<P, FLUX, TS>(T, LT, G, dc, lane, bankA, sR, bankA, bankB);
-        volume_phase_box<P, TS>(T, G, dc, lane, sR, bankA, bankB);
+        volume_phase_box<P, TS, box_rounds<P, TS>(), R, PROJ>(T, G, dc, lane, sR, bankA, bankB);
     }
     // FP32 form: the residual widened to FP64 (bank A is free again)
     double* sRd;
@@ -2043,12 +2153,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     }
 }
 
-template <int P, int FLUX, bool MAPPED, class R = double>
+template <int P, int FLUX, bool MAPPED, class R = double, int PROJ = 0>
 __global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
     int ec[3];
     const unsigned e = block_element_xyz(G, ec);
-    stage_body<P, FLUX, MAPPED, R>(A, *Tp, G, e, ec);
+    stage_body<P, FLUX, MAPPED, R, stage_team<P>(), false, PROJ>(A, *Tp, G, e, ec);
 }
 
 // The closure's corrected RK stage on an affine FP64 box (RK modes only)
@@ -2068,7 +2178,7 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_corr_kernel(StageArgs A
 // tables and RK registers (BatchCase), so every element sees the stage
 // arithmetic of its case's own launch.  Positions past the last element
 // (the padding to a multiple of 8) return at once.
-template <int P, int FLUX, class R = double, int TS = stage_team<P>()>
+template <int P, int FLUX, class R = double, int TS = stage_team<P>(), int PROJ = 0>
 __global__ __launch_bounds__(TS) void stage_batch_kernel(const BatchCase* __restrict__ cases,
                                                                       const unsigned* __restrict__ blk_case,
                                                                       unsigned nreal, int stage,
@@ -2095,7 +2205,7 @@ __global__ __launch_bounds__(TS) void stage_batch_kernel(const BatchCase* __rest
     const KGeom G = C->g;
     int ec[3];
     const unsigned e = brick_element_xyz(G, q - C->first_block, ec);
-    stage_body<P, FLUX, false, R, TS>(A, *Tp, G, e, ec);
+    stage_body<P, FLUX, false, R, TS, false, PROJ>(A, *Tp, G, e, ec);
 }
 
 // dU = dt M^-1 R
@@ -2466,6 +2576,18 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
         } else {
             return hipErrorInvalidValue;
         }
+    } else if (g.fp32 && g.proj == 1) {
+        switch (flux) {
+        case 0: hipLaunchKernelGGL((stage_kernel<P, 0, false, float, 1>), grid, block, 0, st, a, t, g); break;
+        case 1: hipLaunchKernelGGL((stage_kernel<P, 1, false, float, 1>), grid, block, 0, st, a, t, g); break;
+        default: hipLaunchKernelGGL((stage_kernel<P, 2, false, float, 1>), grid, block, 0, st, a, t, g); break;
+        }
+    } else if (g.fp32 && g.proj == 3) {
+        switch (flux) {
+        case 0: hipLaunchKernelGGL((stage_kernel<P, 0, false, float, 3>), grid, block, 0, st, a, t, g); break;
+        case 1: hipLaunchKernelGGL((stage_kernel<P, 1, false, float, 3>), grid, block, 0, st, a, t, g); break;
+        default: hipLaunchKernelGGL((stage_kernel<P, 2, false, float, 3>), grid, block, 0, st, a, t, g); break;
+        }
     } else if (g.fp32) {
         switch (flux) {
         case 0: hipLaunchKernelGGL((stage_kernel<P, 0, false, float>), grid, block, 0, st, a, t, g); break;
@@ -2478,6 +2600,19 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
         case 1: hipLaunchKernelGGL((stage_kernel<P, 1, false>), grid, block, 0, st, a, t, g); break;
         default: hipLaunchKernelGGL((stage_kernel<P, 2, false>), grid, block, 0, st, a, t, g); break;
         }
+    }
+    return hipGetLastError();
+}
+
+template <int P, int TS, int PROJ>
+hipError_t stage_batch_proj(int flux, const BatchCase* cases, const unsigned* blk_case, unsigned nblocks, int stage,
+                            const KTables* t, hipStream_t st) {
+    const dim3 grid((nblocks + 7u) & ~7u), block(TS);
+    const char* base = reinterpret_cast<const char*>(cases);
+    switch (flux) {
+    case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, float, TS, PROJ>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, base); break;
+    case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, float, TS, PROJ>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, base); break;
+    default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, float, TS, PROJ>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, base); break;
     }
     return hipGetLastError();
 }
@@ -2510,8 +2645,11 @@ constexpr int alt_team() {
 }
 
 template <int P>
-hipError_t stage_batch_p(int flux, int fp32, int variant, const BatchCase* cases, const unsigned* blk_case,
+hipError_t stage_batch_p(int flux, int fp32, int proj, int variant, const BatchCase* cases, const unsigned* blk_case,
                          unsigned nblocks, int stage, const KTables* t, hipStream_t st) {
+    // the bf16 projection forms have the default team only
+    if (proj == 1) return stage_batch_proj<P, stage_team<P>(), 1>(flux, cases, blk_case, nblocks, stage, t, st);
+    if (proj == 3) return stage_batch_proj<P, stage_team<P>(), 3>(flux, cases, blk_case, nblocks, stage, t, st);
     if (variant == 1)
         return stage_batch_pt<P, alt_team<P>()>(flux, fp32, cases, blk_case, nblocks, stage, t, st);
     return stage_batch_pt<P, stage_team<P>()>(flux, fp32, cases, blk_case, nblocks, stage, t, st);
@@ -2555,10 +2693,10 @@ hipError_t launch_stage_corrected(int P, int flux, const StageArgs& a, const flo
     return hipGetLastError();
 }
 
-hipError_t launch_stage_batch(int P, int flux, int fp32, int variant, const BatchCase* cases,
+hipError_t launch_stage_batch(int P, int flux, int fp32, int proj, int variant, const BatchCase* cases,
                               const unsigned* blk_case, unsigned nblocks, int stage, const KTables* t, hipStream_t st) {
     if (nblocks == 0) return hipSuccess;
-    DG_DISPATCH_P(P, return stage_batch_p<PP>(flux, fp32, variant, cases, blk_case, nblocks, stage, t, st));
+    DG_DISPATCH_P(P, return stage_batch_p<PP>(flux, fp32, proj, variant, cases, blk_case, nblocks, stage, t, st));
     return hipSuccess;
 }
 

@@ -3,10 +3,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace dg {
 
 constexpr int PMAX = 5;            // highest order with a compiled kernel
 constexpr int NVAR = 5;            // rho, rho u, rho v, rho w, rho E
+// projection matrix of the bf16 form: (p+1)^2 rows and 3 (p+2)^2 columns,
+// padded to 16
+constexpr int KPROJ_ROWS = ((PMAX + 1) * (PMAX + 1) + 15) / 16 * 16;
+constexpr int KPROJ_COLS = (3 * (PMAX + 2) * (PMAX + 2) + 15) / 16 * 16;
 
 // Reference-element operators of one case (1D factors of the tensor bases).
 // Kept in device memory (one copy per solver) and passed to the kernels by
@@ -40,6 +46,13 @@ struct KTables {
     double Dw[PMAX + 2][PMAX + 1];
     float Bw32[PMAX + 2][PMAX + 1];
     float Dw32[PMAX + 2][PMAX + 1];
+    // The x-y part of the weak-gradient projection as one matrix for the
+    // bf16 matrix-core form: Kproj[part][m][kk], row m = j (p+1) + i, column
+    // kk = dir (p+2)^2 + b (p+2) + a, the value Dw[a][i] Bw[b][j] (dir 0, the
+    // x flux), Bw[a][i] Dw[b][j] (dir 1) or Bw[a][i] Bw[b][j] (dir 2),
+    // rounded to bf16 (part 0) with the rounding error in bf16 (part 1);
+    // rows and columns padded with zeros to whole 16 x 16 tiles
+    uint16_t Kproj[2][KPROJ_ROWS][KPROJ_COLS];
 };
 
 // Structured periodic box and physics constants.  dim = 2: nx x ny box of
@@ -59,6 +72,8 @@ struct KGeom {
     // general path (warped box, meshes from files): trilinear elements with
     // face records instead of box neighbours
     int fp32;                       // 1: the stage kernel computes in FP32 (box only)
+    int proj;                       // FP32 form: x-y projection on bf16 matrix cores,
+                                    // 0 none, 1 one bf16 term, 3 hi/lo split (3 terms)
     int mapped;                     // 1: the fields below are used
     int exact_normals;              // 1: face normals at the flux points
     const double* X;                // [E][8][3] node coordinates, node 4k+2j+i
@@ -125,7 +140,7 @@ hipError_t launch_stage_corrected(int P, int flux, const StageArgs& a, const flo
 // group (tables t shared): nblocks = total elements of the group,
 // blk_case[q] = case of traversal position q, stage 0..3 of RK4; variant
 // 0 or 1 picks one of two team sizes per order (batch_team).
-hipError_t launch_stage_batch(int P, int flux, int fp32, int variant, const BatchCase* cases,
+hipError_t launch_stage_batch(int P, int flux, int fp32, int proj, int variant, const BatchCase* cases,
                               const unsigned* blk_case, unsigned nblocks, int stage, const KTables* t, hipStream_t st);
 // threads per element of the batched launch's variant (0 or 1) at order P
 int batch_team(int P, int variant);

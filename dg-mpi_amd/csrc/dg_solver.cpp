@@ -333,7 +333,7 @@ int dg_get_info(const dg_solver* s, dg_info* info) {
         info->num_basis = (c.order + 1) * (c.order + 1) * (c.dim == 3 ? c.order + 1 : 1);
         info->num_state_vars = s->ns;
         info->dim = c.dim;
-        info->precision = c.fp32 ? 32 : 64;
+        info->precision = c.proj ? 16 : (c.fp32 ? 32 : 64);
         for (int d = 0; d < 3; ++d) {
             info->nelem[d] = c.nelem[d];
             info->lower[d] = c.lower[d];
@@ -607,7 +607,7 @@ int dg_set_stream(dg_solver* s, void* stream) {
 struct dg_batch {
     // cases of one (order, basis, flux, precision): one launch per RK stage
     struct Group {
-        int P = 0, basis = 0, flux = 0, fp32 = 0;
+        int P = 0, basis = 0, flux = 0, fp32 = 0, proj = 0;
         unsigned nblocks = 0;
         std::vector<int> members;             // indices into cases
         dg::BatchCase* dcases = nullptr;      // device descriptors
@@ -661,7 +661,7 @@ void upload_group(dg_batch* b, dg_batch::Group& g) {
 hipError_t launch_step(dg_batch* b) {
     for (int st = 0; st < 4; ++st)
         for (auto& g : b->groups) {
-            const hipError_t e = dg::launch_stage_batch(g.P, g.flux, g.fp32, g.variant, g.dcases, g.blk_case,
+            const hipError_t e = dg::launch_stage_batch(g.P, g.flux, g.fp32, g.proj, g.variant, g.dcases, g.blk_case,
                                                         g.nblocks, st, b->cases[g.members[0]]->dtab, b->stream);
             if (e != hipSuccess) return e;
         }
@@ -698,7 +698,7 @@ void tune(dg_batch* b) {
             for (size_t gi = 0; gi < ng; ++gi) {
                 auto& g = b->groups[gi];
                 hip_check(hipEventRecord(ev[gi * 8 + st * 2], b->stream), "hipEventRecord");
-                hip_check(dg::launch_stage_batch(g.P, g.flux, g.fp32, v, g.dcases, g.blk_case, g.nblocks, st,
+                hip_check(dg::launch_stage_batch(g.P, g.flux, g.fp32, g.proj, v, g.dcases, g.blk_case, g.nblocks, st,
                                                  b->cases[g.members[0]]->dtab, b->stream),
                           "batched stage launch");
                 hip_check(hipEventRecord(ev[gi * 8 + st * 2 + 1], b->stream), "hipEventRecord");
@@ -765,10 +765,10 @@ int dg_batch_create(int num_cases, dg_solver* const* cases, dg_batch** out) {
         // tables (they depend on the order and the basis only)
         for (int i = 0; i < num_cases; ++i) {
             const dg_solver* s = b->cases[i];
-            const int flux = (int)s->cfg.flux, fp32 = s->geo.fp32, basis = (int)s->cfg.basis;
+            const int flux = (int)s->cfg.flux, fp32 = s->geo.fp32, basis = (int)s->cfg.basis, proj = s->geo.proj;
             dg_batch::Group* g = nullptr;
             for (auto& q : b->groups)
-                if (q.P == s->P && q.basis == basis && q.flux == flux && q.fp32 == fp32) g = &q;
+                if (q.P == s->P && q.basis == basis && q.flux == flux && q.fp32 == fp32 && q.proj == proj) g = &q;
             if (!g) {
                 b->groups.emplace_back();
                 g = &b->groups.back();
@@ -776,6 +776,7 @@ int dg_batch_create(int num_cases, dg_solver* const* cases, dg_batch** out) {
                 g->basis = basis;
                 g->flux = flux;
                 g->fp32 = fp32;
+                g->proj = proj;
             }
             if ((unsigned long long)g->nblocks + s->geo.E > 0x7ffffff0ULL)
                 throw dg::UnsupportedError("dg_batch_create: more than 2^31 - 1 elements of one order");

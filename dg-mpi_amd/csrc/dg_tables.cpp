@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 
@@ -101,6 +102,20 @@ void basis_1d(Basis b, int p, double x, double* val, double* grad) {
     }
 }
 
+// bf16 of a float, round to nearest even, and back
+static uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float bf16_value(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
 KTables make_tables(const CaseConfig& c) {
     const int p = c.order, n = p + 1, m = p + 2, q = 2 * p + 2;
     if (p < 1 || p > PMAX) throw UnsupportedError("order p = " + std::to_string(p) + " has no GPU kernel (1.." +
@@ -185,6 +200,20 @@ KTables make_tables(const CaseConfig& c) {
         t.phim32[i] = (float)t.phim[i];
         t.phip32[i] = (float)t.phip[i];
     }
+    // x-y projection matrix of the bf16 form (zero padding from the memset)
+    const int mm = m * m;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i)
+            for (int dir = 0; dir < 3; ++dir)
+                for (int b = 0; b < m; ++b)
+                    for (int a = 0; a < m; ++a) {
+                        const double v = dir == 0 ? t.Dw[a][i] * t.Bw[b][j]
+                                                  : (dir == 1 ? t.Bw[a][i] * t.Dw[b][j] : t.Bw[a][i] * t.Bw[b][j]);
+                        const float f = (float)v;
+                        const uint16_t hi = bf16_rne(f);
+                        t.Kproj[0][j * n + i][dir * mm + b * m + a] = hi;
+                        t.Kproj[1][j * n + i][dir * mm + b * m + a] = bf16_rne(f - bf16_value(hi));
+                    }
     return t;
 }
 
@@ -222,6 +251,7 @@ KGeom make_geometry(const CaseConfig& c) {
     g.R = c.gas_constant;
     g.mapped = (c.warp > 0.0 || c.mesh_input) ? 1 : 0;
     g.fp32 = c.fp32 ? 1 : 0;
+    g.proj = c.proj;
     g.exact_normals = c.exact_normals ? 1 : 0;
     g.xcd_remap = (E % 8 == 0 && E >= 8 * 64) ? 1 : 0;
     // Stage-kernel traversal in bricks of up to 8^3 elements (edge: the

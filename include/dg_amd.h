@@ -69,7 +69,9 @@ typedef struct {
                                 rho u, rho v, rho E), nelem[2] = 1           */
     int precision;           /* 64, or 32: Numerics.precision = "FP32" (face
                                 and volume work of the stage in FP32; RK
-                                registers, M^-1 and reductions in FP64)      */
+                                registers, M^-1 and reductions in FP64), or
+                                16: "BF16" / "BF16x3" (the FP32 form with
+                                the x-y projection on bf16 matrix cores)     */
 } dg_info;
 
 /* Parse a TOML case (the keys of src/exec/main.cpp:38-83,

@@ -230,11 +230,19 @@ def test_2d_mesh_input_is_checked_before_the_device():
 
 
 def test_precision_key():
-    """Numerics.precision (a new key): FP64 (default) or FP32, the
-    FP32 form on the affine 3D box only."""
+    """Numerics.precision (a new key): FP64 (default), FP32, BF16 or BF16x3,
+    the reduced forms on the affine 3D box only; Stepper.compensated is a
+    boolean (3D only)."""
     from dg_amd import hip
     base = dg_amd.case_toml(order=2, nelem=(2, 2, 2), timestep=0.01, precision="FP32")
     rc, msg = _create(base)
+    assert rc == (0 if hip.device_count() > 0 else 2), msg
+    for prec in ("BF16", "BF16x3"):
+        rc, msg = _create(base.replace('precision = "FP32"', f'precision = "{prec}"'))
+        assert rc == (0 if hip.device_count() > 0 else 2), msg
+    rc, msg = _create(base.replace("timestep = 0.01", "timestep = 0.01\ncompensated = 1"))
+    assert rc == 1 and "compensated" in msg, msg
+    rc, msg = _create(base.replace("timestep = 0.01", "timestep = 0.01\ncompensated = false"))
     assert rc == (0 if hip.device_count() > 0 else 2), msg
     rc, msg = _create(base.replace('precision = "FP32"', 'precision = "FP16"'))
     assert rc == 1 and "FP16" in msg, msg
