@@ -1428,28 +1428,64 @@ __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<
     sync();
     // 4. Riemann flux at the face points (the + face's element is the left
     //    one); the weights and the side's sign are applied by the lift.
-    //    Each wave handles one direction.
-    team_for<3 * L::RP, TS>(lane, [&](int t) {
-        const int d = t / L::RP, q = t % L::RP;
-        if (q >= 2 * MM) return;
-        const int pt = q % MM, side = q / MM;
-        const R* own = f2 + ((d * 4 + side) * MM + pt) * V;
-        const R* nbv = f2 + ((d * 4 + 2 + side) * MM + pt) * V;
-        const R* pl = side ? own : nbv;
-        const R* pr = side ? nbv : own;
-        R UL[V], UR[V], F[V];
+    //    Where padding each direction to whole waves would leave lanes idle
+    //    (all orders but p = 3), the face states are read in the face-normal
+    //    frame (momentum components cycled so that the normal one comes
+    //    first) and one x-normal solver serves all directions, the flux
+    //    cycled back on the store; otherwise each wave takes one direction.
+    constexpr bool ROT = 3 * L::RP > ((6 * MM + WAVE - 1) / WAVE) * WAVE;
+    if constexpr (ROT) {
+        team_for<6 * MM, TS>(lane, [&](int t) {
+            const int d = t / (2 * MM), q = t - 2 * MM * d;
+            const int pt = q % MM, side = q / MM;
+            const R* own = f2 + ((d * 4 + side) * MM + pt) * V;
+            const R* nbv = f2 + ((d * 4 + 2 + side) * MM + pt) * V;
+            const R* pl = side ? own : nbv;
+            const R* pr = side ? nbv : own;
+            const int c1 = d == 2 ? 0 : d + 1, c2 = d == 0 ? 2 : d - 1;   // (d+1) % 3, (d+2) % 3
+            R UL[V], UR[V], F[V];
+            UL[0] = pl[0];
+            UL[1] = pl[1 + d];
+            UL[2] = pl[1 + c1];
+            UL[3] = pl[1 + c2];
+            UL[4] = pl[4];
+            UR[0] = pr[0];
+            UR[1] = pr[1 + d];
+            UR[2] = pr[1 + c1];
+            UR[3] = pr[1 + c2];
+            UR[4] = pr[4];
+            const R nmag = R(d == 0 ? G.farea[0] : (d == 1 ? G.farea[1] : G.farea[2]));
+            riemann<FLUX, 0>(R(G.gamma), UL, UR, nmag, F);
+            R* o = fh + ((d * 2 + side) * MM + pt) * V;
+            o[0] = F[0];
+            o[1 + d] = F[1];
+            o[1 + c1] = F[2];
+            o[1 + c2] = F[3];
+            o[4] = F[4];
+        });
+    } else {
+        team_for<3 * L::RP, TS>(lane, [&](int t) {
+            const int d = t / L::RP, q = t % L::RP;
+            if (q >= 2 * MM) return;
+            const int pt = q % MM, side = q / MM;
+            const R* own = f2 + ((d * 4 + side) * MM + pt) * V;
+            const R* nbv = f2 + ((d * 4 + 2 + side) * MM + pt) * V;
+            const R* pl = side ? own : nbv;
+            const R* pr = side ? nbv : own;
+            R UL[V], UR[V], F[V];
 #pragma unroll
-        for (int s = 0; s < V; ++s) {
-            UL[s] = pl[s];
-            UR[s] = pr[s];
-        }
-        if (d == 0) riemann<FLUX, 0>(R(G.gamma), UL, UR, R(G.farea[0]), F);
-        else if (d == 1) riemann<FLUX, 1>(R(G.gamma), UL, UR, R(G.farea[1]), F);
-        else riemann<FLUX, 2>(R(G.gamma), UL, UR, R(G.farea[2]), F);
-        R* o = fh + ((d * 2 + side) * MM + pt) * V;
+            for (int s = 0; s < V; ++s) {
+                UL[s] = pl[s];
+                UR[s] = pr[s];
+            }
+            if (d == 0) riemann<FLUX, 0>(R(G.gamma), UL, UR, R(G.farea[0]), F);
+            else if (d == 1) riemann<FLUX, 1>(R(G.gamma), UL, UR, R(G.farea[1]), F);
+            else riemann<FLUX, 2>(R(G.gamma), UL, UR, R(G.farea[2]), F);
+            R* o = fh + ((d * 2 + side) * MM + pt) * V;
 #pragma unroll
-        for (int s = 0; s < V; ++s) o[s] = F[s];
-    });
+            for (int s = 0; s < V; ++s) o[s] = F[s];
+        });
+    }
     sync();
     // 5. lift: transpose of the interpolation with the weights, along t2
     //    then t1
@@ -1851,7 +1887,7 @@ struct StageLoads {
             const int t = r * TS + lane;
             u0[r] = t < NB ? A.U0[base + t] : 0.;
             ac[r] = (t < NB && need_acc) ? A.acc[base + t] : 0.;
-            cm[r] = (t < NB && need_cmp) ? A.cmp[base + t] : 0.;
+            cm[r] = (t < NB && need_cmp) ? (double)A.cmp[base + t] : 0.;
         }
     }
     __device__ __forceinline__ void clear_rk() {
@@ -2147,7 +2183,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
             if (wcmp) {
                 // TwoSum: n_up + err = u0 + inc exactly
                 const double bb = n_up - u0;
-                A.cmp[base + t] = (u0 - (n_up - bb)) + (inc - bb);
+                A.cmp[base + t] = (float)((u0 - (n_up - bb)) + (inc - bb));
             }
         }
     }
@@ -2197,7 +2233,7 @@ __global__ __launch_bounds__(TS) void stage_batch_kernel(const BatchCase* __rest
     A.U0 = reinterpret_cast<const double*>(base + ou0);
     A.Uout = reinterpret_cast<double*>(const_cast<char*>(base) + oout);
     A.acc = reinterpret_cast<double*>(const_cast<char*>(base) + C->off_acc);
-    A.cmp = C->off_cmp >= 0 ? reinterpret_cast<double*>(const_cast<char*>(base) + C->off_cmp) : nullptr;
+    A.cmp = C->off_cmp >= 0 ? reinterpret_cast<float*>(const_cast<char*>(base) + C->off_cmp) : nullptr;
     A.R = nullptr;
     A.dt = C->dt;
     A.alpha = stage == 2 ? 1.0 : 0.5;

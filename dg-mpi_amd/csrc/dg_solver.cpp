@@ -142,14 +142,15 @@ void step_once(dg_solver* s, double dt) {
 
 // a new state starts without carried rounding error
 void clear_cmp(dg_solver* s) {
-    if (s->cmp) hip_check(hipMemsetAsync(s->cmp, 0, s->ncoef * sizeof(double), s->stream), "hipMemset(cmp)");
+    if (s->cmp) hip_check(hipMemsetAsync(s->cmp, 0, s->ncoef * sizeof(float), s->stream), "hipMemset(cmp)");
 }
 
 void free_all(dg_solver* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
-    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->cmp, s->work, s->X, s->imm, s->nev})
+    for (double* p : {s->U0, s->Ua, s->Ub, s->acc, s->work, s->X, s->imm, s->nev})
         if (p) (void)hipFree(p);
+    if (s->cmp) (void)hipFree(s->cmp);
     s->cmp = nullptr;
     if (s->frec) (void)hipFree(s->frec);
     s->frec = nullptr;
@@ -251,8 +252,8 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         hip_check(hipMemcpy(s->dtab, &s->tab, sizeof(dg::KTables), hipMemcpyHostToDevice), "hipMemcpy(tables)");
         hip_check(hipMemsetAsync(s->U0, 0, bytes, s->stream), "hipMemset");
         if (cfg.compensated) {
-            hip_check(hipMalloc(&s->cmp, bytes), "hipMalloc(cmp)");
-            hip_check(hipMemsetAsync(s->cmp, 0, bytes, s->stream), "hipMemset");
+            hip_check(hipMalloc(&s->cmp, s->ncoef * sizeof(float)), "hipMalloc(cmp)");
+            hip_check(hipMemsetAsync(s->cmp, 0, s->ncoef * sizeof(float), s->stream), "hipMemset");
         }
         if (s->geo.mapped) {
             // upload the general-path tables, then form the per-element
@@ -636,7 +637,7 @@ void upload_group(dg_batch* b, dg_batch::Group& g) {
     using namespace dg;
     std::vector<BatchCase> h(g.members.size());
     const char* base = reinterpret_cast<const char*>(g.dcases);
-    auto off = [&](const double* p) { return (long long)(reinterpret_cast<const char*>(p) - base); };
+    auto off = [&](const void* p) { return (long long)(reinterpret_cast<const char*>(p) - base); };
     unsigned first = 0;
     for (size_t k = 0; k < g.members.size(); ++k) {
         const dg_solver* s = b->cases[g.members[k]];

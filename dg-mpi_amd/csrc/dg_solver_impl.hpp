@@ -27,7 +27,7 @@ struct dg_solver {
     double* Ua = nullptr;              // stage states
     double* Ub = nullptr;
     double* acc = nullptr;             // RK accumulator
-    double* cmp = nullptr;             // Stepper.compensated: rounding error of U0
+    float* cmp = nullptr;              // Stepper.compensated: rounding error of U0 (FP32)
     double* work = nullptr;            // per-element reduction output
     double* X = nullptr;               // mapped box: element node coordinates [E][8][3]
     double* imm = nullptr;             // mapped box: inverse mass matrices [E][nb][nb]
