@@ -1265,7 +1265,10 @@ struct BoxLDS {
     static constexpr int X1 = N * N * M * V;
     static constexpr int X2 = N * M * M * V;
     static constexpr int UQ = Dm::MMM * V;
-    static constexpr int QD = 3 * Dm::MMM * V;          // [d][c][b][a][s]
+    // point data: the three flux vectors [d][c][b][a][s] (15 values), or
+    // at p = 5, where those would halve the occupancy, 9 values per point
+    static constexpr bool PF15 = P <= 4;
+    static constexpr int QD = PF15 ? 3 * Dm::MMM * V : 9 * Dm::MMM;
     static constexpr int C = 3 * N * M * M * V;
     static constexpr int BV = 2 * N * N * M * V;
     static constexpr int A = cmax(cmax(U + cmax(cmax(F1, FH), L2), X2), cmax(QD, BV));
@@ -1590,6 +1593,27 @@ __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& 
     sync();
     // point fluxes: G_d[s] = F_d[s] (dxi_d / dx_d) |J| at each volume point,
     // F_d[s] = U_s u_d + p delta_{s, d+1} (E + p in place of E)
+    constexpr bool PF15 = BoxLDS<P>::PF15;
+    if constexpr (!PF15) {
+        // (rho, m, E + p, g_x u, g_y v, g_z w, p), g_d = (dxi_d / dx_d) |J|
+        team_for<MMM, TS>(lane, [&](int pq) {
+            const R* u = Uq + pq * V;
+            R U[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) U[v] = u[v];
+            const R r1 = frcp(U[0]);
+            const R p = pressure_r(R(G.gamma), U, r1);
+            R* o = Qd + pq * 9;
+            o[0] = U[0];
+            o[1] = U[1];
+            o[2] = U[2];
+            o[3] = U[3];
+            o[4] = U[4] + p;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) o[5 + d] = U[1 + d] * r1 * R(G.gvol[d]);
+            o[8] = p;
+        });
+    } else
     team_for<MMM, TS>(lane, [&](int pq) {
         const R* u = Uq + pq * V;
         R U[V];
@@ -1614,10 +1638,25 @@ __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& 
         R xx[N], yy[N], zz[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) xx[k] = yy[k] = zz[k] = 0.;
+        // 9-value points: the pressure enters momentum component s of flux
+        // direction s - 1, scaled by that direction's metric
+        const R ex = s == 1 ? R(G.gvol[0]) : R(0), ey = s == 2 ? R(G.gvol[1]) : R(0),
+                ez = s == 3 ? R(G.gvol[2]) : R(0);
 #pragma unroll
         for (int c = 0; c < M; ++c) {
-            const int o = (c * MM + q) * V + s;
-            const R gx = Qd[o], gy = Qd[MMM * V + o], gz = Qd[2 * MMM * V + o];
+            R gx, gy, gz;
+            if constexpr (PF15) {
+                const int o = (c * MM + q) * V + s;
+                gx = Qd[o];
+                gy = Qd[MMM * V + o];
+                gz = Qd[2 * MMM * V + o];
+            } else {
+                const R* pt = Qd + (c * MM + q) * 9;
+                const R Ws = pt[s], pp = pt[8];
+                gx = fma(Ws, pt[5], ex * pp);
+                gy = fma(Ws, pt[6], ey * pp);
+                gz = fma(Ws, pt[7], ez * pp);
+            }
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 xx[k] = fma(tBw<R>(T, c, k), gx, xx[k]);
