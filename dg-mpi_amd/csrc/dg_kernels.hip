@@ -2114,25 +2114,17 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     const bool rk = A.mode != MODE_RESIDUAL;
     const TaskDec<TS, box_rounds<P, TS>()> dc(lane);
 
-    // lane tables first (their loads are the first to return), then every
-    // global load of the element at once: its block, the neighbour lines
-    // and the RK registers
-    if (lane < PMAX + 2) {
-        LT.w[lane] = tw<R>(T, lane);
-        LT.xv[lane] = R(T.xv[lane]);
-    }
-    if (lane < PMAX + 1) {
-        LT.phim[lane] = tphim<R>(T, lane);
-        LT.phip[lane] = tphip<R>(T, lane);
-        LT.mdiag[lane] = T.minv[lane][lane];
-    }
-    if constexpr (MAPPED) {
-        team_for<168, TS>(lane, [&](int t) {
-            const int which = t / 24;
-            const unsigned el = which == 0 ? e : G.frec[((size_t)e * 6 + (which - 1)) * 2];
-            XL[t] = G.X[(size_t)el * 24 + t % 24];
-        });
-    }
+    // Every global load of the element at once: the lane-table entries
+    // (branch-free, clamped indices, into registers), its block and the
+    // neighbour lines.  The tables go to LDS only after the element's loads
+    // are issued: stored under a lane branch first, each table load was
+    // waited for before the element's loads went out (two serialised
+    // round trips per workgroup).
+    const int tl = lane < PMAX + 2 ? lane : PMAX + 1;
+    const int tp = lane < PMAX + 1 ? lane : PMAX;
+    const R lt_w = tw<R>(T, tl), lt_xv = R(T.xv[tl]);
+    const R lt_phim = tphim<R>(T, tp), lt_phip = tphip<R>(T, tp);
+    const double lt_md = T.minv[tp][tp];
     PF pf;
     BoxNbr<P, TS> bn;
     constexpr bool WAVE_NBR = !MAPPED && P >= 2;
@@ -2146,6 +2138,22 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         bn.load(A.Ust, nbe, lane);
     } else {
         pf.load_neighbours(A.Ust, ec, G, lane);
+    }
+    if (lane < PMAX + 2) {
+        LT.w[lane] = lt_w;
+        LT.xv[lane] = lt_xv;
+    }
+    if (lane < PMAX + 1) {
+        LT.phim[lane] = lt_phim;
+        LT.phip[lane] = lt_phip;
+        LT.mdiag[lane] = lt_md;
+    }
+    if constexpr (MAPPED) {
+        team_for<168, TS>(lane, [&](int t) {
+            const int which = t / 24;
+            const unsigned el = which == 0 ? e : G.frec[((size_t)e * 6 + (which - 1)) * 2];
+            XL[t] = G.X[(size_t)el * 24 + t % 24];
+        });
     }
     sync();
 #pragma unroll

@@ -61,6 +61,10 @@ class Buffer:
         self.ptr = C.c_void_p()
         check(rt().hipMalloc(C.byref(self.ptr), self.nbytes), "hipMalloc")
         check(rt().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
+        # hipMemset on device memory may return before the fill is done, and
+        # it runs on the null stream, which the library's non-blocking
+        # streams do not wait for: finish it before the buffer is handed out
+        synchronize()
 
     @classmethod
     def from_array(cls, a, dtype=np.float64):
@@ -78,6 +82,7 @@ class Buffer:
         assert a.size == self.n
         check(rt().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.n * self.dtype.itemsize, 1),
               "hipMemcpy H2D")
+        synchronize()                          # complete before work on other streams reads it
 
     def download(self, shape=None, count=None):
         """the first `count` elements (default all), optionally reshaped"""
