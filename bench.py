@@ -444,10 +444,15 @@ def closure_bench(args):
     t0 = time.perf_counter()
     train_seeds = list(range(1000, 1000 + args.closure_seeds))
     log = lambda s: print(s, file=sys.stderr, flush=True)     # progress lines
+    # element subsampling keeps each rollout round within 64 M rows (28 GB)
+    stride = 1
+    while closure.rollout_rows(len(train_seeds), n, 24, stride) > 64 * 2 ** 20 and (n ** 3) % (2 * stride) == 0:
+        stride *= 2
     m, hist, rows = closure.train_rollouts(train_seeds, n, fine_order=4, steps=24, tau_steps=4.0,
-                                           epochs=args.closure_epochs, bound=2.0, log=log)
+                                           epochs=args.closure_epochs, bound=2.0, log=log, stride=stride,
+                                           rounds=args.closure_rounds, noise=args.closure_noise)
     t_fit = time.perf_counter() - t0
-    t_gen = None
+    log(f"training done in {t_fit:.1f} s ({rows} rows, element stride {stride})")
     # a priori: held-out rows along plain coarse runs, network vs the zero correction
     Xv, Yv, rv = closure.generate_rollout([1, 2, 3, 4], n, fine_order=4, steps=24, tau_steps=4.0)
     mse_net, mse_zero = closure.a_priori(m, Xv, Yv, rv)
@@ -462,7 +467,8 @@ def closure_bench(args):
     out = {
         "metric": "coarse p=1 density L2 error vs the p=4 run projected to p=1, held-out seeds, t = 0.5",
         "config": {"workload": f"closure_{n}^3_p1_mlp70-128-128-40_bf16", "train_seeds": len(train_seeds),
-                   "rows": int(rows), "epochs": args.closure_epochs},
+                   "rows": int(rows), "element_stride": stride, "epochs": args.closure_epochs,
+                   "rollout_rounds": args.closure_rounds, "feature_jitter": args.closure_noise},
         "train_s": t_fit, "loss_history": hist,
         "target_rms_element_norm": m.target_rms,
         "a_priori_heldout_mse": {"network": mse_net, "zero_correction": mse_zero,
@@ -506,6 +512,8 @@ def main():
     ap.add_argument("--closure", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--closure-seeds", type=int, default=16, help=argparse.SUPPRESS)
     ap.add_argument("--closure-epochs", type=int, default=6, help=argparse.SUPPRESS)
+    ap.add_argument("--closure-rounds", type=int, default=2, help=argparse.SUPPRESS)
+    ap.add_argument("--closure-noise", type=float, default=0.05, help=argparse.SUPPRESS)
     ap.add_argument("--closure-timing", type=int, default=0, help=argparse.SUPPRESS)
     # the large Taylor-Green case (mesh edge; 96 in the configuration)
     ap.add_argument("--tgv", type=int, default=0, help=argparse.SUPPRESS)

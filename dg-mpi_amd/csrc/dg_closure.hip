@@ -64,62 +64,6 @@ __device__ inline float gelu_d(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// H[m][n] = act(sum_k A'[m][k] W[k][n] + b[n]); A' = (A - mean) * scale for
-// the first layer (mean, scale non-null); Z keeps the pre-activation
-template <bool ACT>
-__global__ void dense_fwd(const float* __restrict__ A, const float* __restrict__ mean,
-                          const float* __restrict__ scale, const uint16_t* __restrict__ W,
-                          const float* __restrict__ b, float* __restrict__ Z, float* __restrict__ H, long M, int K,
-                          int N) {
-    const long t = (long)blockIdx.x * TB + threadIdx.x;
-    if (t >= M * N) return;
-    const long m = t / N;
-    const int n = (int)(t % N);
-    const float* a = A + m * K;
-    float acc = b[n];
-    for (int k = 0; k < K; ++k) {
-        const float x = mean ? (a[k] - mean[k]) * scale[k] : a[k];
-        acc = fmaf(x, from_bf16(W[(size_t)k * N + n]), acc);
-    }
-    if (Z) Z[t] = acc;
-    H[t] = ACT ? gelu(acc) : acc;
-}
-
-// G_in[m][k] = (sum_n G[m][n] W[k][n]) gelu'(Z[m][k])
-__global__ void dense_bwd_in(const float* __restrict__ G, const uint16_t* __restrict__ W,
-                             const float* __restrict__ Z, float* __restrict__ Gin, long M, int K, int N) {
-    const long t = (long)blockIdx.x * TB + threadIdx.x;
-    if (t >= M * K) return;
-    const long m = t / K;
-    const int k = (int)(t % K);
-    const float* g = G + m * N;
-    const uint16_t* w = W + (size_t)k * N;
-    float acc = 0.f;
-    for (int n = 0; n < N; ++n) acc = fmaf(g[n], from_bf16(w[n]), acc);
-    Gin[t] = acc * gelu_d(Z[t]);
-}
-
-// dW[k][n] = sum_m A'[m][k] G[m][n], db[n] = sum_m G[m][n] (A' normalised as
-// in dense_fwd when mean is non-null)
-__global__ void dense_grad(const float* __restrict__ A, const float* __restrict__ mean,
-                           const float* __restrict__ scale, const float* __restrict__ G, float* __restrict__ dW,
-                           float* __restrict__ db, long M, int K, int N) {
-    const long t = (long)blockIdx.x * TB + threadIdx.x;
-    if (t >= (long)(K + 1) * N) return;
-    const int k = (int)(t / N), n = (int)(t % N);
-    float acc = 0.f;
-    if (k < K) {
-        for (long m = 0; m < M; ++m) {
-            const float x = mean ? (A[m * K + k] - mean[k]) * scale[k] : A[m * K + k];
-            acc = fmaf(x, G[m * N + n], acc);
-        }
-        dW[(size_t)k * N + n] = acc;
-    } else {
-        for (long m = 0; m < M; ++m) acc += G[m * N + n];
-        db[n] = acc;
-    }
-}
-
 // output gradient of the mean squared error in normalised target units:
 // G = 2 (O - Y / yscale) / (M NO); per-block partial sums of the squared error
 __global__ void loss_grad(const float* __restrict__ O, const float* __restrict__ Y,
@@ -294,6 +238,7 @@ constexpr int K1 = 80;                         // layer-1 inputs padded to whole
 constexpr int L1S = K1 + 4, L2S = NH + 4;      // LDS row strides (bf16), padded off the bank period
 constexpr int N3 = 48;                         // layer-3 outputs padded to whole 16-column tiles
 constexpr int MAX_INFER_BLOCKS = 256;
+constexpr int GCHUNKS = 32;                   // most row chunks of one weight-gradient pass
 
 struct InferLDS {
     uint16_t w1[NH][L1S];                      // W1^T; inputs k >= NF are zero
@@ -449,6 +394,187 @@ __global__ __launch_bounds__(MT) void mlp_infer_kernel(const float* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Training on the matrix cores: FP32 GEMM tiles (v_mfma_f32_16x16x4f32,
+// FP32 operands and accumulation).  In one 16x16x4 step lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16],
+// r = 0..3, of the 16x16 result.  The layer inputs and the gradients stay
+// FP32 and the weights are the bf16 working copy widened exactly, so the
+// arithmetic is that of a plain FP32 network with bf16 weights up to the
+// order of the FP32 sums.
+// ---------------------------------------------------------------------------
+constexpr int GW = 4;                          // waves per training workgroup
+constexpr int WV = 64;                         // lanes per wave
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Z[m][n] = sum_k A'[m][k] W[k][n] + b[n] and H = act(Z) (Z kept when
+// non-null), A' = (A - mean) scale when mean is non-null.  Each wave takes
+// 16 rows and all NJ 16-column tiles of the N outputs (columns >= N masked).
+template <bool ACT, int NJ>
+__global__ __launch_bounds__(GW * WV) void mfma_fwd(const float* __restrict__ A, const float* __restrict__ mean,
+                                                    const float* __restrict__ scale, const uint16_t* __restrict__ W,
+                                                    const float* __restrict__ b, float* __restrict__ Z,
+                                                    float* __restrict__ H, long M, int K, int N) {
+    const int lane = threadIdx.x & (WV - 1), c = lane & 15, kq = lane >> 4;
+    const long m0 = ((long)blockIdx.x * GW + (threadIdx.x / WV)) * 16;
+    if (m0 >= M) return;                       // whole waves; no barrier follows
+    const long ma = m0 + c;
+    const bool rowok = ma < M;
+    f32x4 acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int n = 16 * j + c;
+        const float bn = n < N ? b[n] : 0.f;
+        acc[j] = f32x4{bn, bn, bn, bn};
+    }
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + kq;
+        float a = 0.f;
+        if (rowok && k < K) {
+            a = A[ma * K + k];
+            if (mean) a = (a - mean[k]) * scale[k];
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int n = 16 * j + c;
+            const float w = (k < K && n < N) ? from_bf16(W[(size_t)k * N + n]) : 0.f;
+            acc[j] = mfma4(a, w, acc[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int n = 16 * j + c;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long m = m0 + 4 * kq + r;
+            if (m < M && n < N) {
+                const float z = acc[j][r];
+                if (Z) Z[m * N + n] = z;
+                H[m * N + n] = ACT ? gelu(z) : z;
+            }
+        }
+    }
+}
+
+// Gin[m][k] = (sum_n G[m][n] W[k][n]) gelu'(Z[m][k]) for the Kout = 16 NJ
+// inputs k of a layer with N outputs
+template <int NJ>
+__global__ __launch_bounds__(GW * WV) void mfma_bwd_in(const float* __restrict__ G, const uint16_t* __restrict__ W,
+                                                       const float* __restrict__ Z, float* __restrict__ Gin, long M,
+                                                       int N) {
+    constexpr int KO = 16 * NJ;
+    const int lane = threadIdx.x & (WV - 1), c = lane & 15, kq = lane >> 4;
+    const long m0 = ((long)blockIdx.x * GW + (threadIdx.x / WV)) * 16;
+    if (m0 >= M) return;
+    const long ma = m0 + c;
+    const bool rowok = ma < M;
+    f32x4 acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int n0 = 0; n0 < N; n0 += 4) {
+        const int n = n0 + kq;
+        const float g = (rowok && n < N) ? G[ma * N + n] : 0.f;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int k = 16 * j + c;
+            const float w = n < N ? from_bf16(W[(size_t)k * N + n]) : 0.f;
+            acc[j] = mfma4(g, w, acc[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int k = 16 * j + c;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long m = m0 + 4 * kq + r;
+            if (m < M) Gin[m * KO + k] = acc[j][r] * gelu_d(Z[m * KO + k]);
+        }
+    }
+}
+
+// Weight and bias gradients of a layer with K inputs and N = 16 NJ (or
+// fewer, masked) outputs, partial over one chunk of rows:
+// part[s][k][n] = sum_{m in chunk s} A'[m][k] G[m][n], with the row k = K
+// the bias (A' = 1); A' normalised as in mfma_fwd when mean is non-null.
+// Workgroup (16-row k tile, chunk s); its waves split the chunk's rows and
+// add their tiles in LDS, in a fixed order.
+template <int NJ>
+__global__ __launch_bounds__(GW * WV) void mfma_wgrad(const float* __restrict__ A, const float* __restrict__ mean,
+                                                      const float* __restrict__ scale, const float* __restrict__ G,
+                                                      float* __restrict__ part, long M, int K, int N, long chunk) {
+    __shared__ f32x4 red[GW][NJ][WV];
+    const int wv = threadIdx.x / WV, lane = threadIdx.x & (WV - 1), c = lane & 15, kq = lane >> 4;
+    const int k0 = blockIdx.x * 16, k = k0 + c;
+    const long mb = (long)blockIdx.y * chunk, me = mb + chunk < M ? mb + chunk : M;
+    f32x4 acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long m4 = mb + 4 * wv; m4 < me; m4 += 4 * GW) {
+        const long m = m4 + kq;
+        const bool ok = m < me;
+        float a = 0.f;
+        if (ok && k <= K) {
+            if (k == K) {
+                a = 1.f;
+            } else {
+                a = A[m * K + k];
+                if (mean) a = (a - mean[k]) * scale[k];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int n = 16 * j + c;
+            const float g = (ok && n < N) ? G[m * N + n] : 0.f;
+            acc[j] = mfma4(a, g, acc[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) red[wv][j][lane] = acc[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < NJ * WV; i += GW * WV) {
+        const int j = i / WV, l = i % WV;
+        f32x4 v = red[0][j][l];
+#pragma unroll
+        for (int w = 1; w < GW; ++w) v += red[w][j][l];
+        const int n = 16 * j + (l & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int kk = k0 + 4 * (l >> 4) + r;
+            if (kk <= K && n < N) part[(size_t)blockIdx.y * NP + (size_t)kk * N + n] = v[r];
+        }
+    }
+}
+
+// g[i] = sum_s part[s][i] over the row chunks (fixed order)
+__global__ void grad_reduce(const float* __restrict__ part, float* __restrict__ g, int nchunk) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= NP) return;
+    float v = 0.f;
+    for (int s = 0; s < nchunk; ++s) v += part[(size_t)s * NP + i];
+    g[i] = v;
+}
+
+// acc += (sum of the nb partial squared errors) * inv, one block, fixed order
+__global__ void loss_add(const double* __restrict__ part, long nb, double inv, double* __restrict__ acc) {
+    __shared__ double red[TB];
+    double v = 0.;
+    for (long i = threadIdx.x; i < nb; i += TB) v += part[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = TB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += red[0] * inv;
+}
+
+__global__ void clear_f64_kernel(double* p) { p[threadIdx.x] = 0.; }
+
 // Y = trunc(Df) - Dc + (trunc(Uf) - Uc) / tau, for a coarse state Uc of
 // its own (a coarse run beside the fine one): the tendency mismatch at Uc
 // plus a relaxation of Uc toward the fine state's projection
@@ -503,6 +629,8 @@ struct dg_mlp {
     float *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *O = nullptr, *G3 = nullptr,
           *G2 = nullptr, *G1 = nullptr, *Xb = nullptr, *Yb = nullptr;
     double* part = nullptr;
+    float* gpart = nullptr;      // per-chunk weight-gradient partials [GCHUNKS][NP]
+    double* lacc = nullptr;      // training loss accumulator (device)
     long step = 0;
     std::string err;
 };
@@ -513,8 +641,12 @@ void free_acts(dg_mlp* m) {
     for (float* q : {m->Z1, m->H1, m->Z2, m->H2, m->O, m->G3, m->G2, m->G1, m->Xb, m->Yb})
         if (q) (void)hipFree(q);
     if (m->part) (void)hipFree(m->part);
+    if (m->gpart) (void)hipFree(m->gpart);
+    if (m->lacc) (void)hipFree(m->lacc);
     m->Z1 = m->H1 = m->Z2 = m->H2 = m->O = m->G3 = m->G2 = m->G1 = m->Xb = m->Yb = nullptr;
     m->part = nullptr;
+    m->gpart = nullptr;
+    m->lacc = nullptr;
     m->cap = 0;
 }
 
@@ -549,7 +681,16 @@ void ensure_acts(dg_mlp* m, long rows) {
     al(&m->Xb, rows * NF);
     al(&m->Yb, rows * NO);
     hcheck(hipMalloc(&m->part, (size_t)nblk(rows * NO) * sizeof(double)), "hipMalloc(mlp)");
+    hcheck(hipMalloc(&m->gpart, (size_t)GCHUNKS * NP * sizeof(float)), "hipMalloc(mlp)");
+    hcheck(hipMalloc(&m->lacc, sizeof(double)), "hipMalloc(mlp)");
     m->cap = rows;
+}
+
+// the weight-gradient row chunks of a batch of M rows: at most GCHUNKS
+// chunks of at least 64 rows
+long grad_chunk(long M) {
+    const long c = (M + GCHUNKS - 1) / GCHUNKS;
+    return c < 64 ? 64 : c;
 }
 
 // forward pass of rows X (raw features) into m->O (normalised outputs)
@@ -557,17 +698,19 @@ void forward(dg_mlp* m, const float* X, long rows) {
     hipStream_t st = m->stream;
     const float* mean = m->xs;
     const float* scale = m->xs + NF;
-    hipLaunchKernelGGL(dense_fwd<true>, dim3(nblk(rows * NH)), dim3(TB), 0, st, X, mean, scale, m->pb + OFF_W1,
-                       m->p + OFF_B1, m->Z1, m->H1, rows, NF, NH);
-    hipLaunchKernelGGL(dense_fwd<true>, dim3(nblk(rows * NH)), dim3(TB), 0, st, m->H1, nullptr, nullptr,
-                       m->pb + OFF_W2, m->p + OFF_B2, m->Z2, m->H2, rows, NH, NH);
-    hipLaunchKernelGGL(dense_fwd<false>, dim3(nblk(rows * NO)), dim3(TB), 0, st, m->H2, nullptr, nullptr,
+    const dim3 grid((unsigned)((rows + 16 * GW - 1) / (16 * GW))), block(GW * WV);
+    hipLaunchKernelGGL((mfma_fwd<true, NH / 16>), grid, block, 0, st, X, mean, scale, m->pb + OFF_W1, m->p + OFF_B1,
+                       m->Z1, m->H1, rows, NF, NH);
+    hipLaunchKernelGGL((mfma_fwd<true, NH / 16>), grid, block, 0, st, m->H1, nullptr, nullptr, m->pb + OFF_W2,
+                       m->p + OFF_B2, m->Z2, m->H2, rows, NH, NH);
+    hipLaunchKernelGGL((mfma_fwd<false, (NO + 15) / 16>), grid, block, 0, st, m->H2, nullptr, nullptr,
                        m->pb + OFF_W3, m->p + OFF_B3, nullptr, m->O, rows, NH, NO);
     hcheck(hipGetLastError(), "mlp forward");
 }
 
-// loss and gradient (into m->g) of rows X, Y; returns the mean squared error
-double backward(dg_mlp* m, const float* X, const float* Y, long rows) {
+// loss and gradient (into m->g) of rows X, Y: the batch's mean squared
+// error is added to m->lacc on the device (no host synchronisation)
+void backward(dg_mlp* m, const float* X, const float* Y, long rows) {
     hipStream_t st = m->stream;
     const float* mean = m->xs;
     const float* scale = m->xs + NF;
@@ -575,23 +718,33 @@ double backward(dg_mlp* m, const float* X, const float* Y, long rows) {
     forward(m, X, rows);
     const unsigned nb = nblk(rows * NO);
     hipLaunchKernelGGL(loss_grad, dim3(nb), dim3(TB), 0, st, m->O, Y, ys, m->G3, m->part, rows);
-    hipLaunchKernelGGL(dense_grad, dim3(nblk((long)(NH + 1) * NO)), dim3(TB), 0, st, m->H2, nullptr, nullptr, m->G3,
-                       m->g + OFF_W3, m->g + OFF_B3, rows, NH, NO);
-    hipLaunchKernelGGL(dense_bwd_in, dim3(nblk(rows * NH)), dim3(TB), 0, st, m->G3, m->pb + OFF_W3, m->Z2, m->G2,
-                       rows, NH, NO);
-    hipLaunchKernelGGL(dense_grad, dim3(nblk((long)(NH + 1) * NH)), dim3(TB), 0, st, m->H1, nullptr, nullptr, m->G2,
-                       m->g + OFF_W2, m->g + OFF_B2, rows, NH, NH);
-    hipLaunchKernelGGL(dense_bwd_in, dim3(nblk(rows * NH)), dim3(TB), 0, st, m->G2, m->pb + OFF_W2, m->Z1, m->G1,
-                       rows, NH, NH);
-    hipLaunchKernelGGL(dense_grad, dim3(nblk((long)(NF + 1) * NH)), dim3(TB), 0, st, X, mean, scale, m->G1,
-                       m->g + OFF_W1, m->g + OFF_B1, rows, NF, NH);
+    hipLaunchKernelGGL(loss_add, dim3(1), dim3(TB), 0, st, m->part, (long)nb, 1.0 / (double)(rows * NO), m->lacc);
+    const long chunk = grad_chunk(rows);
+    const unsigned nch = (unsigned)((rows + chunk - 1) / chunk);
+    const dim3 grid((unsigned)((rows + 16 * GW - 1) / (16 * GW))), block(GW * WV);
+    hipLaunchKernelGGL((mfma_wgrad<(NO + 15) / 16>), dim3((NH + 1 + 15) / 16, nch), block, 0, st, m->H2, nullptr,
+                       nullptr, m->G3, m->gpart + OFF_W3, rows, NH, NO, chunk);
+    hipLaunchKernelGGL((mfma_bwd_in<NH / 16>), grid, block, 0, st, m->G3, m->pb + OFF_W3, m->Z2, m->G2, rows, NO);
+    hipLaunchKernelGGL((mfma_wgrad<NH / 16>), dim3((NH + 1 + 15) / 16, nch), block, 0, st, m->H1, nullptr, nullptr,
+                       m->G2, m->gpart + OFF_W2, rows, NH, NH, chunk);
+    hipLaunchKernelGGL((mfma_bwd_in<NH / 16>), grid, block, 0, st, m->G2, m->pb + OFF_W2, m->Z1, m->G1, rows, NH);
+    hipLaunchKernelGGL((mfma_wgrad<NH / 16>), dim3((NF + 1 + 15) / 16, nch), block, 0, st, X, mean, scale, m->G1,
+                       m->gpart + OFF_W1, rows, NF, NH, chunk);
+    hipLaunchKernelGGL(grad_reduce, dim3(nblk(NP)), dim3(TB), 0, st, m->gpart, m->g, (int)nch);
     hcheck(hipGetLastError(), "mlp backward");
-    std::vector<double> h(nb);
-    hcheck(hipMemcpyAsync(h.data(), m->part, nb * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
-    hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-    double s = 0.;
-    for (double v : h) s += v;
-    return s / (double)(rows * NO);
+}
+
+// m->lacc = 0 (a kernel: no memset node on a captured stream)
+void clear_loss(dg_mlp* m) {
+    hipLaunchKernelGGL(clear_f64_kernel, dim3(1), dim3(1), 0, m->stream, m->lacc);
+    hcheck(hipGetLastError(), "clear kernel");
+}
+
+double read_loss(dg_mlp* m) {
+    double v = 0.;
+    hcheck(hipMemcpyAsync(&v, m->lacc, sizeof(double), hipMemcpyDeviceToHost, m->stream), "hipMemcpy");
+    hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+    return v;
 }
 
 template <class F>
@@ -846,7 +999,9 @@ int dg_mlp_gradient(dg_mlp* m, const float* X_dev, const float* Y_dev, long rows
         if (rows < 1) throw std::invalid_argument("rows < 1");
         hcheck(hipSetDevice(m->device), "hipSetDevice");
         ensure_acts(m, rows);
-        const double l = backward(m, X_dev, Y_dev, rows);
+        clear_loss(m);
+        backward(m, X_dev, Y_dev, rows);
+        const double l = read_loss(m);
         if (loss) *loss = l;
         if (grad_host) hcheck(hipMemcpy(grad_host, m->g, NP * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
     });
@@ -863,7 +1018,7 @@ int dg_mlp_train(dg_mlp* m, const float* X_dev, const float* Y_dev, long nrows, 
         if (nidx < batch) throw std::invalid_argument("fewer row indices than one mini-batch (nidx < batch)");
         hcheck(hipSetDevice(m->device), "hipSetDevice");
         ensure_acts(m, batch);
-        double sum = 0.;
+        clear_loss(m);
         long nb = 0;
         for (long i0 = 0; i0 + batch <= nidx; i0 += batch) {
             hipLaunchKernelGGL(gather_rows, dim3(nblk((long)batch * (NF + NO))), dim3(TB), 0, m->stream, X_dev,
@@ -871,7 +1026,7 @@ int dg_mlp_train(dg_mlp* m, const float* X_dev, const float* Y_dev, long nrows, 
             if (noise > 0.f)
                 hipLaunchKernelGGL(jitter_kernel, dim3(nblk((long)batch * NF)), dim3(TB), 0, m->stream, m->Xb,
                                    m->xs + NF, noise, (long)batch * NF, (unsigned long long)m->step * 0x100000001ull);
-            sum += backward(m, m->Xb, m->Yb, batch);
+            backward(m, m->Xb, m->Yb, batch);
             ++nb;
             ++m->step;
             const float b1 = 0.9f, b2 = 0.999f;
@@ -881,7 +1036,7 @@ int dg_mlp_train(dg_mlp* m, const float* X_dev, const float* Y_dev, long nrows, 
                                b1, b2, 1e-8f, c1, c2, NP);
             hcheck(hipGetLastError(), "adam");
         }
-        hcheck(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+        const double sum = read_loss(m);
         if (mean_loss) *mean_loss = nb ? sum / nb : 0.;
     });
 }

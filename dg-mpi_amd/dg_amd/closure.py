@@ -101,22 +101,35 @@ def target_at(fine, Uf_ptr, coarse, Uc_ptr, tau, Y):
     return Y
 
 
-def generate_rollout(seeds, n, fine_order=4, steps=24, tau_steps=4.0, model=None, bound=None, out=None, log=None):
+def rollout_rows(nseeds, n, steps, stride=1):
+    """rows that generate_rollout writes: E / stride per coarse step"""
+    return nseeds * steps * (n ** 3 // stride)
+
+
+def generate_rollout(seeds, n, fine_order=4, steps=24, tau_steps=4.0, model=None, bound=None, out=None, log=None,
+                     stride=1):
     """Training rows along coarse runs beside fine ones: per seed the fine
     run and a coarse p = 1 run (plain, or with `model` and its fallback
     bound) advance together, the coarse step an integer number of fine
     steps; after each coarse step, the features of the coarse state and the
     target of dg_closure_target_at with tau = tau_steps coarse steps.
+    stride > 1 (a divisor of E) keeps every stride-th element of each step,
+    starting at an element that rotates with the step and the seed, so that
+    many seeds fit in memory with their whole time history.
     out = (X, Y, first_row): write into existing buffers from that row."""
     from . import Solver
     E = n ** 3
-    rows = len(seeds) * steps * E
+    assert stride >= 1 and E % stride == 0
+    per_step = E // stride
+    rows = rollout_rows(len(seeds), n, steps, stride)
     if out is None:
         X = hip.Buffer(rows * NF, np.float32)
         Y = hip.Buffer(rows * NO, np.float32)
         r = 0
     else:
         X, Y, r = out
+    Xs = hip.Buffer(E * NF, np.float32) if stride > 1 else None
+    Ys = hip.Buffer(E * NO, np.float32) if stride > 1 else None
     for sd in seeds:
         data = mix_ic(np.random.default_rng(sd))
         with Solver(case(fine_order, n, data, CFL[fine_order])) as f, Solver(case(1, n, data, CFL[1])) as c:
@@ -131,33 +144,50 @@ def generate_rollout(seeds, n, fine_order=4, steps=24, tau_steps=4.0, model=None
                     c.take_time_step(dtc, 1)
                 else:
                     step(c, model, 1e30 if bound is None else bound * model.target_rms, 1, dtc)
-                target_at(f, f.state_device_ptr(), c, c.state_device_ptr(), tau_steps * dtc,
-                          _At(Y.addr + r * NO * 4))
-                features(c, c.state_device_ptr(), _At(X.addr + r * NF * 4))
-                r += E
+                if stride == 1:
+                    target_at(f, f.state_device_ptr(), c, c.state_device_ptr(), tau_steps * dtc,
+                              _At(Y.addr + r * NO * 4))
+                    features(c, c.state_device_ptr(), _At(X.addr + r * NF * 4))
+                else:
+                    target_at(f, f.state_device_ptr(), c, c.state_device_ptr(), tau_steps * dtc, Ys)
+                    features(c, c.state_device_ptr(), Xs)
+                    j = (k * 5 + sd) % stride
+                    X.copy_rows_from(r, Xs, j, NF, per_step, stride)
+                    Y.copy_rows_from(r, Ys, j, NO, per_step, stride)
+                r += per_step
         if log:
             log(f"rollout rows: seed {sd} done ({r} rows)")
     hip.synchronize()
+    if Xs is not None:
+        Xs.free()
+        Ys.free()
     return X, Y, rows
 
 
-def train_rollouts(seeds, n, fine_order=4, steps=24, tau_steps=4.0, epochs=6, bound=2.0, log=None):
-    """Two rounds of rollout training (DAgger, Ross et al. 2011): rows
-    along plain coarse runs, a first fit; rows along coarse runs corrected
-    by that network (fallback bound in target-RMS units), a second fit on
-    both sets.  Returns the second network and the loss histories."""
-    E = n ** 3
-    per = len(seeds) * steps * E
-    X = hip.Buffer(2 * per * NF, np.float32)
-    Y = hip.Buffer(2 * per * NO, np.float32)
-    generate_rollout(seeds, n, fine_order, steps, tau_steps, out=(X, Y, 0), log=log)
-    m1, h1 = fit(X, Y, per, epochs=epochs, batch=4096, lr=2e-3, seed=0, log=log)
-    generate_rollout(seeds, n, fine_order, steps, tau_steps, model=m1, bound=bound, out=(X, Y, per), log=log)
-    m1.close()
-    m2, h2 = fit(X, Y, 2 * per, epochs=epochs, batch=4096, lr=2e-3, seed=1, log=log)
+def train_rollouts(seeds, n, fine_order=4, steps=24, tau_steps=4.0, epochs=6, bound=2.0, log=None, stride=1,
+                   rounds=2, noise=0.0):
+    """Rounds of rollout training (DAgger, Ross et al. 2011): rows along
+    plain coarse runs and a first fit; then, per further round, rows along
+    coarse runs corrected by the latest network (fallback bound in
+    target-RMS units) and a fit on all rows so far (noise: Gaussian feature
+    jitter of the fits, normalised units; stride: element subsampling of
+    generate_rollout).  Returns the last network, the loss histories and
+    the row count."""
+    per = rollout_rows(len(seeds), n, steps, stride)
+    X = hip.Buffer(rounds * per * NF, np.float32)
+    Y = hip.Buffer(rounds * per * NO, np.float32)
+    model, hist = None, []
+    for rd in range(rounds):
+        generate_rollout(seeds, n, fine_order, steps, tau_steps, model=model, bound=bound, out=(X, Y, rd * per),
+                         log=log, stride=stride)
+        m, h = fit(X, Y, (rd + 1) * per, epochs=epochs, batch=4096, lr=2e-3, seed=rd, noise=noise, log=log)
+        if model is not None:
+            model.close()
+        model = m
+        hist.append(h)
     X.free()
     Y.free()
-    return m2, (h1, h2), 2 * per
+    return model, tuple(hist), rounds * per
 
 
 def step(solver, mlp, bound, nsteps, dt=0.0):

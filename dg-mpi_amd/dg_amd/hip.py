@@ -20,6 +20,7 @@ def rt():
         for name, args in {
             "hipMalloc": [C.POINTER(vp), sz], "hipFree": [vp],
             "hipMemcpy": [vp, vp, sz, C.c_int], "hipMemset": [vp, C.c_int, sz],
+            "hipMemcpy2D": [vp, sz, vp, sz, sz, sz, C.c_int],
             "hipDeviceSynchronize": [], "hipGetDeviceCount": [C.POINTER(C.c_int)],
             "hipStreamCreateWithFlags": [C.POINTER(vp), C.c_uint], "hipStreamDestroy": [vp],
             "hipStreamSynchronize": [vp], "hipEventCreate": [C.POINTER(vp)],
@@ -101,6 +102,21 @@ class Buffer:
         src = C.c_void_p(self.ptr.value + start * self.dtype.itemsize)
         check(rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), src, count * self.dtype.itemsize, 2), "hipMemcpy D2H")
         return a
+
+    def copy_rows_from(self, dst_row, src, src_row, row_len, nrows, src_stride=1):
+        """rows dst_row .. dst_row + nrows - 1 (row_len elements each) of this
+        buffer <- rows src_row, src_row + src_stride, ... of buffer src (a
+        device-to-device copy, finished before return)"""
+        isz = self.dtype.itemsize
+        assert src.dtype == self.dtype and nrows >= 0 and src_stride >= 1
+        assert (dst_row + nrows) * row_len <= self.n
+        assert nrows == 0 or (src_row + (nrows - 1) * src_stride + 1) * row_len <= src.n
+        if nrows == 0:
+            return
+        check(rt().hipMemcpy2D(C.c_void_p(self.ptr.value + dst_row * row_len * isz), row_len * isz,
+                               C.c_void_p(src.ptr.value + src_row * row_len * isz), src_stride * row_len * isz,
+                               row_len * isz, nrows, 3), "hipMemcpy2D D2D")
+        synchronize()
 
     def free(self):
         if self.ptr:

@@ -266,3 +266,25 @@ def test_trained_network_gain(gpu):
     assert np.isfinite(cl) and sum(r["closure_fallback_elements"] for r in ev) == 0
     assert cl < 0.85 * p1
     m.close()
+
+
+def test_rollout_element_subsampling(gpu):
+    """generate_rollout with stride s keeps, at coarse step k of seed sd,
+    the rows of elements j, j + s, ... with j = (5 k + sd) mod s of the full
+    rollout: the same runs, bit-identical rows."""
+    n, steps, seeds = 8, 3, [11, 12]
+    X1, Y1, r1 = closure.generate_rollout(seeds, n, fine_order=2, steps=steps)
+    X4, Y4, r4 = closure.generate_rollout(seeds, n, fine_order=2, steps=steps, stride=4)
+    E = n ** 3
+    assert r1 == len(seeds) * steps * E and r4 == r1 // 4
+    Xf, Yf = X1.download((r1, MO.NF)), Y1.download((r1, MO.NO))
+    Xs, Ys = X4.download((r4, MO.NF)), Y4.download((r4, MO.NO))
+    for i, sd in enumerate(seeds):
+        for k in range(steps):
+            j = (5 * k + sd) % 4
+            full = slice((i * steps + k) * E, (i * steps + k + 1) * E)
+            sub = slice((i * steps + k) * (E // 4), (i * steps + k + 1) * (E // 4))
+            assert np.array_equal(Xs[sub], Xf[full][j::4])
+            assert np.array_equal(Ys[sub], Yf[full][j::4])
+    for b in (X1, Y1, X4, Y4):
+        b.free()
