@@ -405,6 +405,7 @@ __global__ __launch_bounds__(MT) void mlp_infer_kernel(const float* __restrict__
 // ---------------------------------------------------------------------------
 constexpr int GW = 4;                          // waves per training workgroup
 constexpr int WV = 64;                         // lanes per wave
+constexpr int NJH = 2;                         // 16-column tiles per wave of the 128-wide layers
 
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -412,7 +413,8 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 
 // Z[m][n] = sum_k A'[m][k] W[k][n] + b[n] and H = act(Z) (Z kept when
 // non-null), A' = (A - mean) scale when mean is non-null.  Each wave takes
-// 16 rows and all NJ 16-column tiles of the N outputs (columns >= N masked).
+// 16 rows and NJ 16-column tiles of the N outputs, the tiles of column
+// group blockIdx.y (columns >= N masked).
 template <bool ACT, int NJ>
 __global__ __launch_bounds__(GW * WV) void mfma_fwd(const float* __restrict__ A, const float* __restrict__ mean,
                                                     const float* __restrict__ scale, const uint16_t* __restrict__ W,
@@ -423,10 +425,11 @@ __global__ __launch_bounds__(GW * WV) void mfma_fwd(const float* __restrict__ A,
     if (m0 >= M) return;                       // whole waves; no barrier follows
     const long ma = m0 + c;
     const bool rowok = ma < M;
+    const int n0 = blockIdx.y * 16 * NJ;
     f32x4 acc[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        const int n = 16 * j + c;
+        const int n = n0 + 16 * j + c;
         const float bn = n < N ? b[n] : 0.f;
         acc[j] = f32x4{bn, bn, bn, bn};
     }
@@ -440,14 +443,14 @@ __global__ __launch_bounds__(GW * WV) void mfma_fwd(const float* __restrict__ A,
         }
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            const int n = 16 * j + c;
+            const int n = n0 + 16 * j + c;
             const float w = (k < K && n < N) ? from_bf16(W[(size_t)k * N + n]) : 0.f;
             acc[j] = mfma4(a, w, acc[j]);
         }
     }
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        const int n = 16 * j + c;
+        const int n = n0 + 16 * j + c;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const long m = m0 + 4 * kq + r;
@@ -460,14 +463,16 @@ __global__ __launch_bounds__(GW * WV) void mfma_fwd(const float* __restrict__ A,
     }
 }
 
-// Gin[m][k] = (sum_n G[m][n] W[k][n]) gelu'(Z[m][k]) for the Kout = 16 NJ
-// inputs k of a layer with N outputs
-template <int NJ>
+// Gin[m][k] = (sum_n G[m][n] W[k][n]) gelu'(Z[m][k]) for the KO inputs k
+// of a layer with N outputs; each wave takes 16 rows and the NJ 16-column
+// tiles of column group blockIdx.y
+template <int KO, int NJ>
 __global__ __launch_bounds__(GW * WV) void mfma_bwd_in(const float* __restrict__ G, const uint16_t* __restrict__ W,
                                                        const float* __restrict__ Z, float* __restrict__ Gin, long M,
                                                        int N) {
-    constexpr int KO = 16 * NJ;
+    static_assert(KO % (16 * NJ) == 0, "whole column groups");
     const int lane = threadIdx.x & (WV - 1), c = lane & 15, kq = lane >> 4;
+    const int k0 = blockIdx.y * 16 * NJ;
     const long m0 = ((long)blockIdx.x * GW + (threadIdx.x / WV)) * 16;
     if (m0 >= M) return;
     const long ma = m0 + c;
@@ -481,14 +486,14 @@ __global__ __launch_bounds__(GW * WV) void mfma_bwd_in(const float* __restrict__
         const float g = (rowok && n < N) ? G[ma * N + n] : 0.f;
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            const int k = 16 * j + c;
+            const int k = k0 + 16 * j + c;
             const float w = n < N ? from_bf16(W[(size_t)k * N + n]) : 0.f;
             acc[j] = mfma4(g, w, acc[j]);
         }
     }
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-        const int k = 16 * j + c;
+        const int k = k0 + 16 * j + c;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const long m = m0 + 4 * kq + r;
@@ -497,19 +502,19 @@ __global__ __launch_bounds__(GW * WV) void mfma_bwd_in(const float* __restrict__
     }
 }
 
-// Weight and bias gradients of a layer with K inputs and N = 16 NJ (or
-// fewer, masked) outputs, partial over one chunk of rows:
-// part[s][k][n] = sum_{m in chunk s} A'[m][k] G[m][n], with the row k = K
-// the bias (A' = 1); A' normalised as in mfma_fwd when mean is non-null.
-// Workgroup (16-row k tile, chunk s); its waves split the chunk's rows and
-// add their tiles in LDS, in a fixed order.
+// Weight and bias gradients of a layer with K inputs and N outputs, partial
+// over one chunk of rows: part[s][k][n] = sum_{m in chunk s} A'[m][k]
+// G[m][n], with the row k = K the bias (A' = 1); A' normalised as in
+// mfma_fwd when mean is non-null.  Workgroup (16-row k tile, chunk s,
+// column group of NJ 16-column tiles; columns >= N masked); its waves split
+// the chunk's rows and add their tiles in LDS, in a fixed order.
 template <int NJ>
 __global__ __launch_bounds__(GW * WV) void mfma_wgrad(const float* __restrict__ A, const float* __restrict__ mean,
                                                       const float* __restrict__ scale, const float* __restrict__ G,
                                                       float* __restrict__ part, long M, int K, int N, long chunk) {
     __shared__ f32x4 red[GW][NJ][WV];
     const int wv = threadIdx.x / WV, lane = threadIdx.x & (WV - 1), c = lane & 15, kq = lane >> 4;
-    const int k0 = blockIdx.x * 16, k = k0 + c;
+    const int k0 = blockIdx.x * 16, k = k0 + c, n0 = blockIdx.z * 16 * NJ;
     const long mb = (long)blockIdx.y * chunk, me = mb + chunk < M ? mb + chunk : M;
     f32x4 acc[NJ];
 #pragma unroll
@@ -528,7 +533,7 @@ __global__ __launch_bounds__(GW * WV) void mfma_wgrad(const float* __restrict__ 
         }
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            const int n = 16 * j + c;
+            const int n = n0 + 16 * j + c;
             const float g = (ok && n < N) ? G[m * N + n] : 0.f;
             acc[j] = mfma4(a, g, acc[j]);
         }
@@ -541,7 +546,7 @@ __global__ __launch_bounds__(GW * WV) void mfma_wgrad(const float* __restrict__ 
         f32x4 v = red[0][j][l];
 #pragma unroll
         for (int w = 1; w < GW; ++w) v += red[w][j][l];
-        const int n = 16 * j + (l & 15);
+        const int n = n0 + 16 * j + (l & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int kk = k0 + 4 * (l >> 4) + r;
@@ -698,12 +703,13 @@ void forward(dg_mlp* m, const float* X, long rows) {
     hipStream_t st = m->stream;
     const float* mean = m->xs;
     const float* scale = m->xs + NF;
-    const dim3 grid((unsigned)((rows + 16 * GW - 1) / (16 * GW))), block(GW * WV);
-    hipLaunchKernelGGL((mfma_fwd<true, NH / 16>), grid, block, 0, st, X, mean, scale, m->pb + OFF_W1, m->p + OFF_B1,
+    const unsigned rt = (unsigned)((rows + 16 * GW - 1) / (16 * GW));
+    const dim3 gh(rt, NH / (16 * NJH)), go(rt, (NO + 15) / 16), block(GW * WV);
+    hipLaunchKernelGGL((mfma_fwd<true, NJH>), gh, block, 0, st, X, mean, scale, m->pb + OFF_W1, m->p + OFF_B1,
                        m->Z1, m->H1, rows, NF, NH);
-    hipLaunchKernelGGL((mfma_fwd<true, NH / 16>), grid, block, 0, st, m->H1, nullptr, nullptr, m->pb + OFF_W2,
+    hipLaunchKernelGGL((mfma_fwd<true, NJH>), gh, block, 0, st, m->H1, nullptr, nullptr, m->pb + OFF_W2,
                        m->p + OFF_B2, m->Z2, m->H2, rows, NH, NH);
-    hipLaunchKernelGGL((mfma_fwd<false, (NO + 15) / 16>), grid, block, 0, st, m->H2, nullptr, nullptr,
+    hipLaunchKernelGGL((mfma_fwd<false, 1>), go, block, 0, st, m->H2, nullptr, nullptr,
                        m->pb + OFF_W3, m->p + OFF_B3, nullptr, m->O, rows, NH, NO);
     hcheck(hipGetLastError(), "mlp forward");
 }
@@ -721,15 +727,15 @@ void backward(dg_mlp* m, const float* X, const float* Y, long rows) {
     hipLaunchKernelGGL(loss_add, dim3(1), dim3(TB), 0, st, m->part, (long)nb, 1.0 / (double)(rows * NO), m->lacc);
     const long chunk = grad_chunk(rows);
     const unsigned nch = (unsigned)((rows + chunk - 1) / chunk);
-    const dim3 grid((unsigned)((rows + 16 * GW - 1) / (16 * GW))), block(GW * WV);
-    hipLaunchKernelGGL((mfma_wgrad<(NO + 15) / 16>), dim3((NH + 1 + 15) / 16, nch), block, 0, st, m->H2, nullptr,
-                       nullptr, m->G3, m->gpart + OFF_W3, rows, NH, NO, chunk);
-    hipLaunchKernelGGL((mfma_bwd_in<NH / 16>), grid, block, 0, st, m->G3, m->pb + OFF_W3, m->Z2, m->G2, rows, NO);
-    hipLaunchKernelGGL((mfma_wgrad<NH / 16>), dim3((NH + 1 + 15) / 16, nch), block, 0, st, m->H1, nullptr, nullptr,
-                       m->G2, m->gpart + OFF_W2, rows, NH, NH, chunk);
-    hipLaunchKernelGGL((mfma_bwd_in<NH / 16>), grid, block, 0, st, m->G2, m->pb + OFF_W2, m->Z1, m->G1, rows, NH);
-    hipLaunchKernelGGL((mfma_wgrad<NH / 16>), dim3((NF + 1 + 15) / 16, nch), block, 0, st, X, mean, scale, m->G1,
-                       m->gpart + OFF_W1, rows, NF, NH, chunk);
+    const dim3 gh((unsigned)((rows + 16 * GW - 1) / (16 * GW)), NH / (16 * NJH)), block(GW * WV);
+    hipLaunchKernelGGL((mfma_wgrad<1>), dim3((NH + 1 + 15) / 16, nch, (NO + 15) / 16), block, 0, st, m->H2,
+                       nullptr, nullptr, m->G3, m->gpart + OFF_W3, rows, NH, NO, chunk);
+    hipLaunchKernelGGL((mfma_bwd_in<NH, NJH>), gh, block, 0, st, m->G3, m->pb + OFF_W3, m->Z2, m->G2, rows, NO);
+    hipLaunchKernelGGL((mfma_wgrad<NJH>), dim3((NH + 1 + 15) / 16, nch, NH / (16 * NJH)), block, 0, st, m->H1,
+                       nullptr, nullptr, m->G2, m->gpart + OFF_W2, rows, NH, NH, chunk);
+    hipLaunchKernelGGL((mfma_bwd_in<NH, NJH>), gh, block, 0, st, m->G2, m->pb + OFF_W2, m->Z1, m->G1, rows, NH);
+    hipLaunchKernelGGL((mfma_wgrad<NJH>), dim3((NF + 1 + 15) / 16, nch, NH / (16 * NJH)), block, 0, st, X, mean,
+                       scale, m->G1, m->gpart + OFF_W1, rows, NF, NH, chunk);
     hipLaunchKernelGGL(grad_reduce, dim3(nblk(NP)), dim3(TB), 0, st, m->gpart, m->g, (int)nch);
     hcheck(hipGetLastError(), "mlp backward");
 }
