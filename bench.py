@@ -187,10 +187,12 @@ def batch_bench(args):
     0.04 for p = 1..4 (the dt = CFL h / max(|u| + c) rule is stable to these
     values over the whole Mach range; at Mach 0.2-0.5 CFL 0.1 is not at
     p = 3) -- advanced
-    by dg_batch_step (one launch per order and RK stage, steps replayed from
-    a captured HIP graph), against the same cases stepped one after the
-    other with dg_step; then the same batch in the FP32 form against the
-    FP64 one.  Prints one JSON line."""
+    by dg_batch_step (one launch per order and RK stage, the order groups as
+    concurrent branches of a captured HIP graph, team size and element
+    bricks per group from the autotuner), against the same cases stepped
+    one after the other with dg_step; then the same batch in the FP32 and
+    BF16 forms against the FP64 one.  Per group: its rate run alone and its
+    fraction of the HBM roofline.  Prints one JSON line."""
     import numpy as np
     import dg_amd
     from dg_amd import hip
@@ -218,17 +220,34 @@ def batch_bench(args):
         return out
 
     dof = sum(n ** 3 * (p + 1) ** 3 for n, p, *_ in cases)
-    K, W = args.steps, args.warmup
-    tunings = {}
+    K, W = args.steps, max(args.warmup, 6)
+    tunings, tiles = {}, {}
 
     def batch_time(solvers, key):
-        # the warm-up call (4 steps at least) tunes the team sizes
+        # the warm-up call (6 steps at least) tunes team sizes and bricks
         with dg_amd.Batch(solvers) as b:
-            b.step(max(W, 4))
+            b.step(W)
+            hip.synchronize()
             t0 = time.perf_counter()
             b.step(K)
             tunings[key] = b.tuning()
+            tiles[key] = b.tiles()
             return (time.perf_counter() - t0) / K
+
+    def groups_of(key):
+        # per group: its cases run alone at the chosen candidate (the tuning
+        # timings), rate and fraction of the HBM roofline at 170 B per DOF
+        # per stage (the FP64 register traffic of a stage)
+        out = []
+        for (o, team, a, b2), (cap, nc, ne, ms) in zip(tunings[key], tiles[key]):
+            g_dof = sum(n ** 3 * (p + 1) ** 3 for n, p, *_ in cases if p == o)
+            best = min(ms)
+            out.append({"order": o, "cases": nc, "elements": ne, "dof": g_dof, "team": team,
+                        "brick": f"{2 ** cap}^3 max", "ms_per_step_alone": best,
+                        "ms_candidates": {f"team{c & 1}_brick{2 ** (3 - (c >> 1))}": m for c, m in enumerate(ms)},
+                        "dof_updates_per_s": g_dof / (best / 4 * 1e-3),
+                        "hbm_roofline_frac": ALG_BYTES_PER_DOF * g_dof / (best / 4 * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        return out
 
     s64 = make(None)
     t64 = batch_time(s64, "fp64")
@@ -240,10 +259,17 @@ def batch_bench(args):
                     for a, b in zip(s32, s64)])
     for s in s32:
         s.close()
+    sb = make("BF16")
+    tb = batch_time(sb, "bf16")
+    eb = np.array([s.get_error()[0] for s in sb])
+    db = np.array([float(np.max(np.abs(a.get_state() - b.get_state())) / np.max(np.abs(b.get_state())))
+                   for a, b in zip(sb, s64)])
+    for s in sb:
+        s.close()
     # the same FP64 cases one after the other (dg_step per case), same steps
     seq = make(None)
     for s in seq:
-        s.take_time_step(0.0, max(W, 4))
+        s.take_time_step(0.0, W)
     hip.synchronize()
     t0 = time.perf_counter()
     for s in seq:
@@ -255,21 +281,25 @@ def batch_bench(args):
     for s in s64 + seq:
         s.close()
     ratio = e32 / e64
+    ratio_b = eb / e64
     out = {
         "metric": "aggregate DOF-updates/s per RK stage, batched uneven periodic vortex cases",
-        "value": dof / (t64 / 4), "unit": "DOF-updates/s", "n_gpus": 1, "steps": K, "warmup": max(W, 4),
+        "value": dof / (t64 / 4), "unit": "DOF-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": t64 * 1e3, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"batch_{n_cases}_vortex_cases_n8-32_p1-4_legendre_roe_rk4_cfl0.1-0.04",
                    "seed": args.seed, "cases": n_cases, "elements": int(sum(n ** 3 for n, *_ in cases)),
                    "dof": int(dof), "orders": {str(q): sum(1 for c in cases if c[1] == q) for q in range(1, 5)}},
         "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t64,
-        "tuning": {k: [{"order": o, "team": t, "ms_default_team": a, "ms_alternate_team": b2}
-                       for o, t, a, b2 in v] for k, v in tunings.items()},
+        "groups": {k: groups_of(k) for k in tunings},
         "max_rel_diff_vs_sequential": same,
         "fp32_form": {"value": dof / (t32 / 4), "ms_per_step": t32 * 1e3, "speedup_vs_fp64": t64 / t32,
                       "rho_l2_error_ratio_vs_fp64": {"median": float(np.median(ratio)), "max": float(np.max(ratio)),
                                                      "min": float(np.min(ratio))},
                       "state_max_rel_diff_vs_fp64": {"median": float(np.median(d32)), "max": float(np.max(d32))}},
+        "bf16_form": {"value": dof / (tb / 4), "ms_per_step": tb * 1e3, "speedup_vs_fp64": t64 / tb,
+                      "rho_l2_error_ratio_vs_fp64": {"median": float(np.median(ratio_b)),
+                                                     "max": float(np.max(ratio_b)), "min": float(np.min(ratio_b))},
+                      "state_max_rel_diff_vs_fp64": {"median": float(np.median(db)), "max": float(np.max(db))}},
         "rho_l2_error_fp64": {"median": float(np.median(e64)), "max": float(np.max(e64)),
                               "min": float(np.min(e64))},
     }

@@ -2285,7 +2285,9 @@ __global__ __launch_bounds__(TS) void stage_batch_kernel(const BatchCase* __rest
     A.dt = C->dt;
     A.alpha = stage == 2 ? 1.0 : 0.5;
     A.mode = stage == 0 ? MODE_RK_FIRST : (stage == 3 ? MODE_RK_LAST : MODE_RK_MID);
-    const KGeom G = C->g;
+    // the geometry is read in place (a local copy is indexed at run time in
+    // places and would be placed in scratch memory)
+    const KGeom& G = C->g;
     int ec[3];
     const unsigned e = brick_element_xyz(G, q - C->first_block, ec);
     stage_body<P, FLUX, false, R, TS, false, PROJ>(A, *Tp, G, e, ec);

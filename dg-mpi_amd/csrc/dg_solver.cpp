@@ -1,4 +1,6 @@
 // C ABI of the DG residual + RK path (include/dg_amd.h).
+#include <cstdlib>
+#include <algorithm>
 #include "dg_amd.h"
 
 #include <hip/hip_runtime.h>
@@ -605,6 +607,13 @@ int dg_set_stream(dg_solver* s, void* stream) {
 // Batches of independent cases (dg_batch_*)
 // ---------------------------------------------------------------------------
 
+// Tuning candidates of a batch group: team-size variant (dg::batch_team) x
+// cap on the log2 edge of the element bricks each case is traversed in
+// (8^3 or 4^3 bricks, never larger than the case's own default)
+constexpr int NCAPS = 2;
+constexpr int BRICK_CAPS[NCAPS] = {3, 2};
+constexpr int NCAND = 2 * NCAPS;
+
 struct dg_batch {
     // cases of one (order, basis, flux, precision): one launch per RK stage
     struct Group {
@@ -614,7 +623,11 @@ struct dg_batch {
         dg::BatchCase* dcases = nullptr;      // device descriptors
         unsigned* blk_case = nullptr;         // device: case (within the group) of each block
         int variant = 0;                      // team-size variant of the launch (dg::batch_team)
-        double ms[2] = {0.0, 0.0};            // measured time of one RK4 step per variant
+        int cap = BRICK_CAPS[0];              // log2 edge cap of the element bricks of its cases
+        double ms[2] = {0.0, 0.0};            // one RK4 step with each team variant at the chosen cap
+        double ms_cand[NCAND] = {};           // one RK4 step per tuning candidate (team variant, cap)
+        hipStream_t stream = nullptr;         // the group's branch of the captured step
+        hipEvent_t done = nullptr;            // end of that branch
     };
     std::vector<dg_solver*> cases;
     std::vector<Group> groups;
@@ -623,6 +636,7 @@ struct dg_batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;               // one RK4 step of every group
     hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr;                // start of a step (captured)
     bool tuned = false;                       // variants chosen by timing
     std::string err;
 };
@@ -650,23 +664,40 @@ void upload_group(dg_batch* b, dg_batch::Group& g) {
         c.off_cmp = s->cmp ? off(s->cmp) : -1;
         c.dt = s->dt;
         c.g = s->geo;
+        // the traversal bricks of the group's cap (the elements and their
+        // arithmetic are the same in any traversal order)
+        for (int d = 0; d < 3; ++d) c.g.tile[d] = std::min(s->geo.tile[d], g.cap);
         c.first_block = first;
         first += s->geo.E;
     }
-    hip_check(hipMemcpy(g.dcases, h.data(), h.size() * sizeof(BatchCase), hipMemcpyHostToDevice),
+    // ordered on the batch stream (the group branches of a step fork from it)
+    hip_check(hipMemcpyAsync(g.dcases, h.data(), h.size() * sizeof(BatchCase), hipMemcpyHostToDevice, b->stream),
               "hipMemcpy(batch cases)");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
 }
 
-// one RK4 step of every group, launched on the batch stream (captured or
-// eager)
+// one RK4 step of every group, to be captured: the groups are independent
+// (disjoint cases), so each runs its four stages on its own branch (a
+// stream forked from the batch stream and joined back at the end), and a
+// group's stages overlap the other groups' launches
+// (DG_BATCH_SERIAL=1 in the environment launches every group on the batch
+// stream instead, one after the other: the same work, for comparisons)
 hipError_t launch_step(dg_batch* b) {
-    for (int st = 0; st < 4; ++st)
-        for (auto& g : b->groups) {
-            const hipError_t e = dg::launch_stage_batch(g.P, g.flux, g.fp32, g.proj, g.variant, g.dcases, g.blk_case,
-                                                        g.nblocks, st, b->cases[g.members[0]]->dtab, b->stream);
-            if (e != hipSuccess) return e;
-        }
-    return hipSuccess;
+    static const bool serial = std::getenv("DG_BATCH_SERIAL") != nullptr;
+    hipError_t e = hipEventRecord(b->fork, b->stream);
+    for (size_t gi = 0; gi < b->groups.size() && e == hipSuccess; ++gi) {
+        auto& g = b->groups[gi];
+        const bool branch = gi > 0 && !serial;
+        hipStream_t st = branch ? g.stream : b->stream;
+        if (branch) e = hipStreamWaitEvent(st, b->fork, 0);
+        for (int k = 0; k < 4 && e == hipSuccess; ++k)
+            e = dg::launch_stage_batch(g.P, g.flux, g.fp32, g.proj, g.variant, g.dcases, g.blk_case, g.nblocks, k,
+                                       b->cases[g.members[0]]->dtab, st);
+        if (e == hipSuccess && branch) e = hipEventRecord(g.done, st);
+    }
+    for (size_t gi = 1; gi < b->groups.size() && e == hipSuccess && !serial; ++gi)
+        e = hipStreamWaitEvent(b->stream, b->groups[gi].done, 0);
+    return e;
 }
 
 void capture_step(dg_batch* b) {
@@ -682,19 +713,30 @@ void capture_step(dg_batch* b) {
     hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
 }
 
-// Autotuning: four eager RK4 steps with the variants 0, 1, 0, 1 for every
-// group, each group's launches bracketed by events; the second measurement
-// of each variant counts (the first pays for code loading).  Each output
-// value is computed by one thread with the same operations whatever the
-// team size (each variant being compiled on its own, multiply-add fusion
-// may differ: round-off), so these are ordinary steps of the cases.
+// Autotuning: TUNE_STEPS eager RK4 steps, each group's launches bracketed
+// by events: first one step with each team variant at the default cap (the
+// first launch of a kernel pays for code loading; not counted), then one
+// step with each candidate (team variant, brick cap); each group keeps its
+// fastest candidate.  Each output value is computed by one thread with the
+// same operations whatever the team size or the traversal order (each
+// variant being compiled on its own, multiply-add fusion may differ:
+// round-off), so these are ordinary steps of the cases.
+constexpr int TUNE_STEPS = 2 + NCAND;
+
 void tune(dg_batch* b) {
     const size_t ng = b->groups.size();
     std::vector<hipEvent_t> ev(ng * 8);
     for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-    for (int k = 0; k < 4; ++k) {
-        const int v = k & 1;
-        for (auto& g : b->groups) g.variant = v;
+    for (int k = 0; k < TUNE_STEPS; ++k) {
+        const int cand = k < 2 ? k : k - 2;            // candidate = variant + 2 cap index
+        const int v = cand & 1, ci = cand >> 1;
+        for (auto& g : b->groups) {
+            g.variant = v;
+            if (g.cap != BRICK_CAPS[ci]) {
+                g.cap = BRICK_CAPS[ci];
+                upload_group(b, g);
+            }
+        }
         for (int st = 0; st < 4; ++st)
             for (size_t gi = 0; gi < ng; ++gi) {
                 auto& g = b->groups[gi];
@@ -705,6 +747,7 @@ void tune(dg_batch* b) {
                 hip_check(hipEventRecord(ev[gi * 8 + st * 2 + 1], b->stream), "hipEventRecord");
             }
         hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        if (k < 2) continue;
         for (size_t gi = 0; gi < ng; ++gi) {
             double ms = 0.0;
             for (int st = 0; st < 4; ++st) {
@@ -713,11 +756,20 @@ void tune(dg_batch* b) {
                           "hipEventElapsedTime");
                 ms += t;
             }
-            b->groups[gi].ms[v] = ms;
+            b->groups[gi].ms_cand[cand] = ms;
         }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    for (auto& g : b->groups) g.variant = g.ms[1] < g.ms[0] ? 1 : 0;
+    for (auto& g : b->groups) {
+        int best = 0;
+        for (int c = 1; c < NCAND; ++c)
+            if (g.ms_cand[c] < g.ms_cand[best]) best = c;
+        g.variant = best & 1;
+        g.cap = BRICK_CAPS[best >> 1];
+        g.ms[0] = g.ms_cand[(best >> 1) * 2];
+        g.ms[1] = g.ms_cand[(best >> 1) * 2 + 1];
+        upload_group(b, g);
+    }
     b->tuned = true;
 }
 
@@ -729,7 +781,10 @@ void free_batch(dg_batch* b) {
     for (auto& g : b->groups) {
         if (g.dcases) (void)hipFree(g.dcases);
         if (g.blk_case) (void)hipFree(g.blk_case);
+        if (g.stream) (void)hipStreamDestroy(g.stream);
+        if (g.done) (void)hipEventDestroy(g.done);
     }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -785,6 +840,12 @@ int dg_batch_create(int num_cases, dg_solver* const* cases, dg_batch** out) {
             g->nblocks += s->geo.E;
         }
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+        for (size_t gi = 1; gi < b->groups.size(); ++gi) {
+            // default priority, as every stream of this library
+            hip_check(hipStreamCreateWithFlags(&b->groups[gi].stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&b->groups[gi].done, hipEventDisableTiming), "hipEventCreate");
+        }
         for (auto& g : b->groups) {
             hip_check(hipMalloc(&g.dcases, g.members.size() * sizeof(dg::BatchCase)), "hipMalloc(batch cases)");
             hip_check(hipMalloc(&g.blk_case, (size_t)g.nblocks * sizeof(unsigned)), "hipMalloc(batch blocks)");
@@ -831,12 +892,12 @@ int dg_batch_step(dg_batch* b, int nsteps) {
             for (size_t i = 0; i < b->cases.size(); ++i) b->dts[i] = b->cases[i]->dt;
         }
         int n0 = 0;
-        if (!b->tuned && nsteps >= 4) {
-            // the first four steps of a long enough call pick each group's
-            // team size, then the step is captured again with the choices
+        if (!b->tuned && nsteps >= TUNE_STEPS) {
+            // the first steps of a long enough call pick each group's team
+            // size and bricks, then the step is captured again with them
             tune(b);
             capture_step(b);
-            n0 = 4;
+            n0 = TUNE_STEPS;
         }
         for (int n = n0; n < nsteps; ++n) hip_check(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
         hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
@@ -869,6 +930,24 @@ int dg_batch_tuning(const dg_batch* b, int* num_groups, int* order, int* team, d
             if (team) team[i] = dg::batch_team(g.P, g.variant);
             if (ms_default) ms_default[i] = b->tuned ? g.ms[0] : 0.0;
             if (ms_alternate) ms_alternate[i] = b->tuned ? g.ms[1] : 0.0;
+        }
+    });
+}
+
+int dg_batch_tuning_tiles(const dg_batch* b, int* num_groups, int* num_candidates, int* brick_cap, int* cases,
+                          long* elements, double* ms_candidates) {
+    return guarded_to(b ? const_cast<dg_batch*>(b)->err : g_batch_error, [&] {
+        need(b, "batch");
+        need(num_groups, "num_groups");
+        *num_groups = (int)b->groups.size();
+        if (num_candidates) *num_candidates = NCAND;
+        for (size_t i = 0; i < b->groups.size(); ++i) {
+            const auto& g = b->groups[i];
+            if (brick_cap) brick_cap[i] = g.cap;
+            if (cases) cases[i] = (int)g.members.size();
+            if (elements) elements[i] = (long)g.nblocks;
+            if (ms_candidates)
+                for (int c = 0; c < NCAND; ++c) ms_candidates[i * NCAND + c] = b->tuned ? g.ms_cand[c] : 0.0;
         }
     });
 }

@@ -36,7 +36,7 @@ EXPORTS = ["dg_create", "dg_create_mesh", "dg_destroy", "dg_last_error", "dg_get
            "dg_set_state", "dg_get_state", "dg_state_device_ptr", "dg_residual",
            "dg_apply_inv_mass", "dg_step", "dg_solve", "dg_l2_error", "dg_integrals",
            "dg_max_wavespeed", "dg_set_time", "dg_set_stream", "dg_batch_create", "dg_batch_step",
-           "dg_batch_destroy", "dg_batch_last_error", "dg_batch_tuning", "dg_mlp_create", "dg_mlp_destroy", "dg_mlp_last_error",
+           "dg_batch_destroy", "dg_batch_last_error", "dg_batch_tuning", "dg_batch_tuning_tiles", "dg_mlp_create", "dg_mlp_destroy", "dg_mlp_last_error",
            "dg_mlp_num_params", "dg_mlp_get_params", "dg_mlp_set_params", "dg_mlp_set_scaling", "dg_mlp_forward",
            "dg_mlp_gradient", "dg_mlp_train", "dg_closure_features", "dg_closure_target", "dg_closure_target_at",
            "dg_closure_step", "dg_closure_step_noalloc"]
@@ -94,6 +94,7 @@ def lib():
             "dg_batch_step": [vp, C.c_int],
             "dg_batch_destroy": [vp],
             "dg_batch_tuning": [vp, C.POINTER(C.c_int), vp, vp, vp, vp],
+            "dg_batch_tuning_tiles": [vp, C.POINTER(C.c_int), vp, vp, vp, vp, vp],
             "dg_mlp_create": [C.c_ulonglong, C.POINTER(vp)],
             "dg_mlp_destroy": [vp],
             "dg_mlp_num_params": [C.POINTER(C.c_long)],
@@ -320,6 +321,18 @@ class Batch:
         a, b = (C.c_double * n.value)(), (C.c_double * n.value)()
         lib().dg_batch_tuning(self._h, C.byref(n), o, t, a, b)
         return [(o[i], t[i], a[i], b[i]) for i in range(n.value)]
+
+    def tiles(self):
+        """per group: (brick cap in use, cases, elements, [ms per step of
+        each candidate: team variant + 2 cap index]; zeros before tuning)"""
+        n, nc = C.c_int(), C.c_int()
+        rc = lib().dg_batch_tuning_tiles(self._h, C.byref(n), C.byref(nc), None, None, None, None)
+        if rc != 0:
+            raise DGError(rc, lib().dg_batch_last_error(self._h).decode())
+        cap, nca = (C.c_int * n.value)(), (C.c_int * n.value)()
+        el, ms = (C.c_long * n.value)(), (C.c_double * (n.value * nc.value))()
+        lib().dg_batch_tuning_tiles(self._h, C.byref(n), C.byref(nc), cap, nca, el, ms)
+        return [(cap[i], nca[i], el[i], list(ms[i * nc.value:(i + 1) * nc.value])) for i in range(n.value)]
 
     def close(self):
         if self._h:

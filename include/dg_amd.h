@@ -203,15 +203,27 @@ int dg_batch_destroy(dg_batch* b);
 /* Last error of the batch (NULL: of the last failed dg_batch_create on this
  * thread).  Never returns NULL. */
 const char* dg_batch_last_error(const dg_batch* b);
-/* Each (order, basis, flux, precision) group's launch has two team sizes;
- * the first dg_batch_step call with nsteps >= 4 runs its first four steps
- * with the two alternately, keeps the faster per group (the second timing
- * of each) and captures the step again.  This reports, per group, the
- * order, the team size in use and the two timings (ms per RK4 step of the
- * group; 0 before tuning).  Arrays of *num_groups entries; NULL arrays:
- * only the count. */
+/* Each (order, basis, flux, precision) group's launch has tuning
+ * candidates: two team sizes x two element-brick caps (the cases of the
+ * group are traversed in bricks of at most 8^3 or 4^3 elements).  The
+ * first dg_batch_step call with nsteps >= 6 runs its first six steps as
+ * tuning steps (one with each team size, not counted: code loading; then
+ * one with each candidate), keeps each group's fastest candidate and
+ * captures the step again.  This reports, per group, the order, the team
+ * size in use and the two team sizes' timings at the chosen cap (ms per
+ * RK4 step of the group run alone; 0 before tuning).  Arrays of
+ * *num_groups entries; NULL arrays: only the count. */
 int dg_batch_tuning(const dg_batch* b, int* num_groups, int* order, int* team, double* ms_default,
                     double* ms_alternate);
+
+/* The tile side of the tuning (no reference counterpart): per group the
+ * brick cap in use (log2 of the largest brick edge), the number of cases and
+ * elements, and the timing of every candidate (ms per RK4 step of the group
+ * run alone; ms_candidates[group * num_candidates + team + 2 cap_index],
+ * cap_index 0 = 8^3 bricks, 1 = 4^3; 0 before tuning).  NULL arrays are
+ * skipped. */
+int dg_batch_tuning_tiles(const dg_batch* b, int* num_groups, int* num_candidates, int* brick_cap, int* cases,
+                          long* elements, double* ms_candidates);
 
 /* Learned closure for coarse runs (no reference counterpart).  A per-element
  * MLP -- 70 features -> 128 -> 128 -> 40, GELU, BF16 weights with FP32

@@ -99,11 +99,13 @@ def test_batch_input_checks(gpu):
 
 
 def test_batch_tuning_keeps_results(gpu):
-    """A call with nsteps >= 4 times the two team sizes of every group on its
-    first four steps and keeps the faster; the cases still end where their
-    own dg_step takes them, to round-off (a team size of its own compiles
-    to its own multiply-add fusion: ~1e-16 relative in FP64, ~1e-8 for the
-    FP32-form case)."""
+    """A call with nsteps >= 6 times, on its first six steps, the candidates
+    of every group (two team sizes x bricks of at most 8^3 or 4^3 elements)
+    and keeps the fastest; the cases still end where their own dg_step takes
+    them, to round-off (a team size of its own compiles to its own
+    multiply-add fusion: ~1e-16 relative in FP64, ~1e-8 for the FP32-form
+    case; the traversal order changes nothing), also on replays of the
+    re-captured step, whose groups run as concurrent graph branches."""
     cases = [CASES[0], CASES[2], CASES[3], CASES[5], CASES[6]]
     batch_s = [dg_amd.Solver(toml(c)) for c in cases]
     solo_s = [dg_amd.Solver(toml(c)) for c in cases]
@@ -118,6 +120,14 @@ def test_batch_tuning_keeps_results(gpu):
         for order, team, t0, t1 in tun:
             assert t0 > 0.0 and t1 > 0.0
             assert team in (64, 128, 256, 512)
+        tiles = b.tiles()
+        print("tiles (brick cap, cases, elements, ms per candidate):", tiles, flush=True)
+        assert sum(t[1] for t in tiles) == len(cases)
+        for (cap, nc, ne, ms), (order, team, t0, t1) in zip(tiles, tun):
+            assert cap in (3, 2) and len(ms) == 4 and min(ms) > 0.0
+            ci = 0 if cap == 3 else 1
+            assert min(ms) == min(ms[2 * ci], ms[2 * ci + 1])      # the chosen cap holds the fastest candidate
+            assert (t0, t1) == (ms[2 * ci], ms[2 * ci + 1])
         for s in solo_s:
             s.take_time_step(0.0, 6)
         tol = [1e-14 if c[6] is None else 2e-6 for c in cases]
