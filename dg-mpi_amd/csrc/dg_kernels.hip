@@ -355,22 +355,26 @@ __device__ __forceinline__ void riemann_roe(R gam, const R* UL, const R* UR, R n
     const R l2 = fabs(unt);
     const R l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
     const R w1 = l1 * a1, w5 = l5 * a5, w2 = l2 * a2, ws = l2 * rt;
+    // sum over the waves of w_k r_k, grouped by eigenvector component:
+    // the acoustic pair enters as (w1 + w5) and (w5 - w1) c
+    const R wsum = w1 + w5, wdc = (w5 - w1) * ct;
     R Dv[V];
-    Dv[0] = w1 + w5 + w2;
+    Dv[0] = wsum + w2;
     R udu = R(0.);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const R cn = (d == D) ? ct : R(0.0);
         const R dut = (d == D) ? (uR[d] - uL[d]) - dun : (uR[d] - uL[d]);
-        Dv[1 + d] = w1 * (ut[d] - cn) + w5 * (ut[d] + cn) + w2 * ut[d] + ws * dut;
-        udu += ut[d] * dut;
+        const R v = fma(Dv[0], ut[d], ws * dut);
+        Dv[1 + d] = (d == D) ? v + wdc : v;
+        udu = fma(ut[d], dut, udu);
     }
-    Dv[4] = w1 * (Ht - unt * ct) + w5 * (Ht + unt * ct) + w2 * R(0.5) * q2 + ws * udu;
+    Dv[4] = fma(wsum, Ht, fma(wdc, unt, fma(w2 * R(0.5), q2, ws * udu)));
     R FL[V], FR[V];
     flux_axis<D>(UL, r1L, pL, FL);
     flux_axis<D>(UR, r1R, pR, FR);
+    const R hn = R(0.5) * nmag;
 #pragma unroll
-    for (int s = 0; s < V; ++s) F[s] = nmag * (R(0.5) * (FL[s] + FR[s]) - R(0.5) * Dv[s]);
+    for (int s = 0; s < V; ++s) F[s] = hn * ((FL[s] + FR[s]) - Dv[s]);
 }
 
 template <int FLUX, int D, class R>
@@ -561,16 +565,18 @@ __device__ __forceinline__ void riemann_roe_n(double gam, const double* UL, cons
     const double l2 = fabs(unt);
     const double l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
     const double w1 = l1 * a1, w5 = l5 * a5, w2 = l2 * a2, ws = l2 * rt;
+    // grouped by eigenvector component, as in riemann_roe
+    const double wsum = w1 + w5, wdc = (w5 - w1) * ct;
     double Dv[V];
-    Dv[0] = w1 + w5 + w2;
+    Dv[0] = wsum + w2;
     double udu = 0.;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double dut = (uR[d] - uL[d]) - dun * n[d];
-        Dv[1 + d] = w1 * (ut[d] - ct * n[d]) + w5 * (ut[d] + ct * n[d]) + w2 * ut[d] + ws * dut;
-        udu += ut[d] * dut;
+        Dv[1 + d] = fma(Dv[0], ut[d], fma(wdc, n[d], ws * dut));
+        udu = fma(ut[d], dut, udu);
     }
-    Dv[4] = w1 * (Ht - unt * ct) + w5 * (Ht + unt * ct) + w2 * 0.5 * q2 + ws * udu;
+    Dv[4] = fma(wsum, Ht, fma(wdc, unt, fma(w2 * 0.5, q2, ws * udu)));
     double FL[V], FR[V];
     flux_dot(UL, r1L, pL, n, FL);
     flux_dot(UR, r1R, pR, n, FR);
