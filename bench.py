@@ -180,7 +180,7 @@ def io_bench(s, path):
 
 def batch_bench(args):
     """Batched uneven cases: 256 independent periodic vortex problems (seed
-    args.seed) with n^3 elements, n uniform in 8..32, order p in {1..4},
+    args.seed) with n^3 elements, n uniform in 8..32 (8..args.batch_max_n), order p in {1..4},
     vortex centre (x, y) in [-2, 2]^2 (the vortex is z-extruded),
     vortex strength beta in [1, 5], free-stream Mach number in [0.2, 0.8]
     along (1, 1, 0) (rho = p = 1), Legendre, Roe, RK4 at CFL 0.1, 0.1, 0.06,
@@ -201,7 +201,7 @@ def batch_bench(args):
     n_cases = args.batch
     cases = []
     for _ in range(n_cases):
-        n = int(rng.integers(8, 33))
+        n = int(rng.integers(8, args.batch_max_n + 1))
         p = int(rng.integers(1, 5))
         beta = float(rng.uniform(1.0, 5.0))
         mach = float(rng.uniform(0.2, 0.8))
@@ -286,7 +286,7 @@ def batch_bench(args):
         "metric": "aggregate DOF-updates/s per RK stage, batched uneven periodic vortex cases",
         "value": dof / (t64 / 4), "unit": "DOF-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": t64 * 1e3, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batch_{n_cases}_vortex_cases_n8-32_p1-4_legendre_roe_rk4_cfl0.1-0.04",
+        "config": {"workload": f"batch_{n_cases}_vortex_cases_n8-{args.batch_max_n}_p1-4_legendre_roe_rk4_cfl0.1-0.04",
                    "seed": args.seed, "cases": n_cases, "elements": int(sum(n ** 3 for n, *_ in cases)),
                    "dof": int(dof), "orders": {str(q): sum(1 for c in cases if c[1] == q) for q in range(1, 5)}},
         "sequential_ms_per_step": t_seq * 1e3, "speedup_vs_sequential": t_seq / t64,
@@ -538,6 +538,7 @@ def main():
     # batched uneven cases (dg_batch_*): number of cases and their seed
     ap.add_argument("--batch", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--seed", type=int, default=7, help=argparse.SUPPRESS)
+    ap.add_argument("--batch-max-n", type=int, default=32, help=argparse.SUPPRESS)
     # learned closure: coarse mesh edge, training seeds, epochs
     ap.add_argument("--closure", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--closure-seeds", type=int, default=16, help=argparse.SUPPRESS)
