@@ -433,7 +433,6 @@ __global__ __launch_bounds__(GW * WV) void mfma_fwd(const float* __restrict__ A,
         const float bn = n < N ? b[n] : 0.f;
         acc[j] = f32x4{bn, bn, bn, bn};
     }
-#pragma unroll 4
     for (int k0 = 0; k0 < K; k0 += 4) {
         const int k = k0 + kq;
         float a = 0.f;
@@ -480,7 +479,6 @@ __global__ __launch_bounds__(GW * WV) void mfma_bwd_in(const float* __restrict__
     f32x4 acc[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
     for (int n0 = 0; n0 < N; n0 += 4) {
         const int n = n0 + kq;
         const float g = (rowok && n < N) ? G[ma * N + n] : 0.f;
