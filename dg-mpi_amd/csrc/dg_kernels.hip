@@ -2825,6 +2825,24 @@ int batch_team(int P, int variant) {
     }
 }
 
+// Zero n bytes of device memory (n a multiple of 4) with a kernel on the
+// stream: a fill kernel, unlike a memset, is a plain kernel node when the
+// caller captures the stream into a graph
+__global__ void zero_kernel(uint32_t* __restrict__ p, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
+    if (bytes % 4 != 0) return hipErrorInvalidValue;
+    const size_t n = bytes / 4;
+    if (n == 0) return hipSuccess;
+    const size_t want = (n + 255) / 256;
+    const unsigned grid = (unsigned)(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(256), 0, st, static_cast<uint32_t*>(p), n);
+    return hipGetLastError();
+}
+
 hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,
                            const KGeom& g, hipStream_t st) {
     if (g.dim == 2) {

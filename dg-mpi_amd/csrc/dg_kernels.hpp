@@ -145,6 +145,8 @@ hipError_t launch_stage_batch(int P, int flux, int fp32, int proj, int variant, 
                               const unsigned* blk_case, unsigned nblocks, int stage, const KTables* t, hipStream_t st);
 // threads per element of the batched launch's variant (0 or 1) at order P
 int batch_team(int P, int variant);
+// zero `bytes` (a multiple of 4) of device memory with a kernel on st
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
 hipError_t launch_inv_mass(int P, double dt, const double* R, double* dU, const KTables* t,
                            const KGeom& g, hipStream_t st);
 hipError_t launch_init_state(int P, double time, double* U, const KTables* t, const KGeom& g,

@@ -144,7 +144,7 @@ void step_once(dg_solver* s, double dt) {
 
 // a new state starts without carried rounding error
 void clear_cmp(dg_solver* s) {
-    if (s->cmp) hip_check(hipMemsetAsync(s->cmp, 0, s->ncoef * sizeof(float), s->stream), "hipMemset(cmp)");
+    if (s->cmp) hip_check(dg::launch_zero(s->cmp, s->ncoef * sizeof(float), s->stream), "zero kernel (cmp)");
 }
 
 void free_all(dg_solver* s) {
@@ -252,10 +252,10 @@ int create(const char* toml_text, const MeshInput* mesh, dg_solver** out) {
         hip_check(hipMalloc(&s->work, (size_t)s->geo.E * 6 * sizeof(double)), "hipMalloc(work)");
         hip_check(hipMalloc(&s->dtab, sizeof(dg::KTables)), "hipMalloc(tables)");
         hip_check(hipMemcpy(s->dtab, &s->tab, sizeof(dg::KTables), hipMemcpyHostToDevice), "hipMemcpy(tables)");
-        hip_check(hipMemsetAsync(s->U0, 0, bytes, s->stream), "hipMemset");
+        hip_check(dg::launch_zero(s->U0, bytes, s->stream), "zero kernel (U0)");
         if (cfg.compensated) {
             hip_check(hipMalloc(&s->cmp, s->ncoef * sizeof(float)), "hipMalloc(cmp)");
-            hip_check(hipMemsetAsync(s->cmp, 0, s->ncoef * sizeof(float), s->stream), "hipMemset");
+            hip_check(dg::launch_zero(s->cmp, s->ncoef * sizeof(float), s->stream), "zero kernel (cmp)");
         }
         if (s->geo.mapped) {
             // upload the general-path tables, then form the per-element
