@@ -1259,19 +1259,24 @@ struct ProjDims {
 
 // Row strides (in elements) of the box path's LDS arrays slab, f1, l1, l2
 // (face phase) and X1, Bv (volume phase): each array is rows of one line
-// of points x V variables.  For p = 3 in FP64 a row is padded to the
-// smallest length >= its own that is = V (mod 16): the task of lane t is
-// (row q, variable s) with t = q V + s, so the address of a row's element
-// is then t + const (mod 16) and the 16 lanes of each pass of a 64-bit LDS
-// access fall on 16 distinct bank pairs (32 banks of 4 bytes).  Measured:
-// the f1 row alone at 37 instead of 25 doubles takes 75 bank-conflict
-// cycles per element off (635 -> 560).  The other orders and FP32 keep
-// their rows unpadded.
+// of points x V variables.  For p = 3 a row is padded to the smallest
+// length >= its own that is = V modulo the slots of one pass of an LDS
+// access (32 banks of 4 bytes: 16 lanes over 16 double slots for FP64, 32
+// lanes over 32 word slots for FP32): the task of lane t is (row q,
+// variable s) with t = q V + s, so the address of a row's element is then
+// t + const modulo the slots and the lanes of a pass fall on distinct
+// banks.  Measured: the f1 row alone at 37 instead of 25 doubles takes 75
+// bank-conflict cycles per element off (635 -> 560); all six rows padded,
+// 635 -> 433 in FP64 and 422 -> 357 in FP32.  The other orders keep their
+// rows unpadded.
 template <int P, class R>
 struct BoxPad {
     static constexpr int N = P + 1, M = P + 2;
-    static constexpr bool ON = P == 3 && sizeof(R) == 8;
-    static constexpr int pad(int n) { return ON ? n + ((V - n) % 16 + 16) % 16 : n; }
+    static constexpr bool ON = P == 3;
+    // FP64: 16 lanes per pass over 16 double slots; FP32: 32 lanes per pass
+    // over 32 word slots
+    static constexpr int SLOTS = sizeof(R) == 8 ? 16 : 32;
+    static constexpr int pad(int n) { return ON ? n + ((V - n) % SLOTS + SLOTS) % SLOTS : n; }
     static constexpr int SL = pad(N * V), F1 = pad(M * V), L1 = pad(M * V), L2 = pad(N * V);
     static constexpr int X1 = pad(M * V), BV = pad(M * V);
 };
