@@ -1265,14 +1265,17 @@ struct ProjDims {
 // lanes over 32 word slots for FP32): the task of lane t is (row q,
 // variable s) with t = q V + s, so the address of a row's element is then
 // t + const modulo the slots and the lanes of a pass fall on distinct
-// banks.  Measured: the f1 row alone at 37 instead of 25 doubles takes 75
-// bank-conflict cycles per element off (635 -> 560); all six rows padded,
-// 635 -> 433 in FP64 and 422 -> 357 in FP32.  The other orders keep their
-// rows unpadded.
+// banks.  Measured at p = 3: the f1 row alone at 37 instead of 25 doubles
+// takes 75 bank-conflict cycles per element off (635 -> 560); all six rows
+// padded, 635 -> 433 in FP64 and 422 -> 357 in FP32.  FP64 pads from
+// p = 2 up (the padded rows fit the LDS that the point data and the c-step
+// output already take at p = 2 and 4; measured per launch at 64^3: p = 2
+// -1.1 %, p = 4 -2.3 %, p = 5 -1.3 %, while p = 1 ran 3 % slower padded);
+// FP32 at p = 3 only.
 template <int P, class R>
 struct BoxPad {
     static constexpr int N = P + 1, M = P + 2;
-    static constexpr bool ON = P == 3;
+    static constexpr bool ON = (sizeof(R) == 8 && P >= 2) || P == 3;
     // FP64: 16 lanes per pass over 16 double slots; FP32: 32 lanes per pass
     // over 32 word slots
     static constexpr int SLOTS = sizeof(R) == 8 ? 16 : 32;
