@@ -84,6 +84,21 @@ def measured_traffic():
     return None
 
 
+def measured_copy_gbs():
+    """Copy bandwidth this GPU reaches: dg-mpi_amd/bin/hbm_copy (a plain
+    device-to-device copy kernel, 1 GiB read and 1 GiB written per launch),
+    run as a child process; None where the tool is not built."""
+    import subprocess
+    exe = os.path.join(ROOT, "dg-mpi_amd", "bin", "hbm_copy")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe, "1024", "20"], capture_output=True, text=True, timeout=120)
+        return float(json.loads(r.stdout.strip().splitlines()[-1])["GBps"]) if r.returncode == 0 else None
+    except (OSError, ValueError, KeyError, IndexError, subprocess.TimeoutExpired):
+        return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -659,6 +674,11 @@ def main():
         "finite": finite,
         "wall_s": wall,
     }
+    if headline:
+        # the second denominator: the copy rate a plain kernel reaches on this GPU
+        copy = measured_copy_gbs()
+        out["roofline"]["copy_measured_GBps"] = copy
+        out["roofline"]["frac_vs_copy"] = achieved / copy if copy else None
     gpu_step = None
     if dim == 3 and not args.warp and not args.fp32 and not args.precision and not args.no_fp32:
         # the reduced-precision forms on the same workload, against this FP64

@@ -281,6 +281,9 @@ CaseConfig parse_case(const std::string& toml_text, bool mesh_input) {
     c.fp32 = prec >= 1;
     c.proj = prec == 2 ? 1 : (prec == 3 ? 3 : 0);
     if (c.fp32 && dim != 3) throw UnsupportedError("Numerics.precision = FP32 with Physics.dim = 2 (3D box only)");
+    c.subtract_mean = get_bool(d, "Numerics", "subtract_mean", c.fp32);
+    if (c.subtract_mean && !c.fp32)
+        throw UnsupportedError("Numerics.subtract_mean applies to Numerics.precision = FP32, BF16 or BF16x3");
     if (c.basis == Basis::LegendreHex && c.order > 7)
         throw InputError("Legendre polynomial P = " + std::to_string(c.order) + " not implemented.");
 

@@ -57,6 +57,10 @@ struct CaseConfig {
     // "BF16" (1) / "BF16x3" (3): the FP32 form with the x-y projection on
     // bf16 matrix cores (one term / a hi-lo split with three products)
     int proj = 0;
+    // subtract_mean = true (new key; the default of the reduced-precision
+    // forms): the FP32 stage subtracts the domain-mean state from the
+    // coefficients before rounding them and forms deviation fluxes
+    bool subtract_mean = false;
     // [Mesh]: structured periodic box, type = "PeriodicBox", nelem, lower,
     // upper; warp > 0 displaces the nodes periodically (trilinear,
     // non-affine elements); normals = "reference" (the reference's

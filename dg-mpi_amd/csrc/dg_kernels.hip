@@ -72,6 +72,7 @@ struct LaneTabT {
     R phip[PMAX + 1];        // phi_i(+1)
     double mdiag[PMAX + 1];  // diagonal of the 1D inverse mass matrix
     R xv[PMAX + 2];          // GL nodes of the (p+2)-point rule
+    double one[sizeof(R) == 4 ? PMAX + 1 : 1];   // FP32 form: the constant 1 in the 1D basis
 };
 using LaneTab = LaneTabT<double>;
 
@@ -375,6 +376,247 @@ __device__ __forceinline__ void riemann_roe(R gam, const R* UL, const R* UR, R n
     const R hn = R(0.5) * nmag;
 #pragma unroll
     for (int s = 0; s < V; ++s) F[s] = hn * ((FL[s] + FR[s]) - Dv[s]);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 form: the state as a deviation from a reference state
+// ---------------------------------------------------------------------------
+// The FP32 form subtracts a constant reference state Ur (conserved
+// variables, chosen by the host: the domain mean) from the coefficients in
+// FP64 before rounding them to FP32; traces and interpolations then carry
+// U' = U - Ur, and every flux is evaluated as the difference F(Ur + U') -
+// F(Ur) by the identities below, exact in U'.  The constant flux F(Ur) has
+// no weak divergence (the volume term and the face terms of a constant
+// flux cancel under the (p+2)-point rule), so the residual is unchanged,
+// while FP32 rounding scales with the deviation U' instead of the state.
+// Ur = 0 gives the plain fluxes.
+// uref[s] of the reference state by selects on values (a dynamic index,
+// or a select of the members' addresses, would copy the kernel-argument
+// struct to scratch memory)
+__device__ __forceinline__ double uref_of(const KGeom& G, int s) {
+    const double u0 = G.uref[0], u1 = G.uref[1], u2 = G.uref[2], u3 = G.uref[3], u4 = G.uref[4];
+    return s == 0 ? u0 : (s == 1 ? u1 : (s == 2 ? u2 : (s == 3 ? u3 : u4)));
+}
+
+template <class R>
+struct SplitRef {
+    R rho, v[3], m[3], E, p, H;    // m = rho v, p = (gamma - 1)(E - m.v / 2), H = E + p
+    // momentum components cycled so that direction d comes first (the
+    // face-normal frame of the x-normal Riemann solver)
+    __device__ __forceinline__ SplitRef cycled(int d) const {
+        // selects on values, not on the members' addresses
+        const R v0 = v[0], v1 = v[1], v2 = v[2], m0 = m[0], m1 = m[1], m2 = m[2];
+        SplitRef r;
+        r.rho = rho;
+        r.E = E;
+        r.p = p;
+        r.H = H;
+        r.v[0] = d == 0 ? v0 : (d == 1 ? v1 : v2);
+        r.v[1] = d == 0 ? v1 : (d == 1 ? v2 : v0);
+        r.v[2] = d == 0 ? v2 : (d == 1 ? v0 : v1);
+        r.m[0] = d == 0 ? m0 : (d == 1 ? m1 : m2);
+        r.m[1] = d == 0 ? m1 : (d == 1 ? m2 : m0);
+        r.m[2] = d == 0 ? m2 : (d == 1 ? m0 : m1);
+        return r;
+    }
+};
+
+template <class R>
+__device__ __forceinline__ SplitRef<R> split_ref(const KGeom& G) {
+    SplitRef<R> r;
+    r.rho = R(G.uref[0]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        r.v[d] = R(G.vref[d]);
+        r.m[d] = R(G.uref[1 + d]);
+    }
+    r.E = R(G.uref[4]);
+    r.p = R(G.pref);
+    r.H = R(G.uref[4] + G.pref);
+    return r;
+}
+
+// Point quantities of Ur + U' from the deviation U'.
+template <class R>
+struct SplitPt {
+    R rho, r1;        // density, 1 / density
+    R v[3], dv[3];    // velocity and its deviation v - vr = (m' - vr rho') / rho
+    R dp, p;          // pressure deviation and pressure
+};
+
+template <class R>
+__device__ __forceinline__ void split_point(R gm1, const SplitRef<R>& rf, const R* U, SplitPt<R>& q) {
+    q.rho = rf.rho + U[0];
+    q.r1 = frcp(q.rho);
+    R ke = R(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        q.dv[d] = fma(-rf.v[d], U[0], U[1 + d]) * q.r1;
+        q.v[d] = rf.v[d] + q.dv[d];
+        // m.v - mr.vr = m'.v + mr.v'
+        ke = fma(U[1 + d], q.v[d], fma(rf.m[d], q.dv[d], ke));
+    }
+    q.dp = gm1 * fma(R(-0.5), ke, U[4]);
+    q.p = rf.p + q.dp;
+}
+
+// F_D(Ur + U') - F_D(Ur): m'_D; m'_i v_D + mr_i v'_D + p' delta_iD;
+// (E' + p') v_D + Hr v'_D
+template <int D, class R>
+__device__ __forceinline__ void flux_axis_split(const SplitRef<R>& rf, const R* U, const SplitPt<R>& q, R* F) {
+    const R vn = q.v[D], dvn = q.dv[D];
+    F[0] = U[1 + D];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const R t = fma(U[1 + i], vn, rf.m[i] * dvn);
+        F[1 + i] = (i == D) ? t + q.dp : t;
+    }
+    F[4] = fma(U[4] + q.dp, vn, rf.H * dvn);
+}
+
+// The three numerical fluxes minus F(Ur).N from deviation states: the
+// formulas of riemann_llf, riemann_hllc and riemann_roe, with the central
+// fluxes, the jumps and the star-state corrections formed from U'.
+template <int D, class R>
+__device__ __forceinline__ void riemann_llf_split(R gam, const SplitRef<R>& rf, const R* UL, const R* UR, R nmag,
+                                                  R* F) {
+    SplitPt<R> qL, qR;
+    split_point(gam - R(1), rf, UL, qL);
+    split_point(gam - R(1), rf, UR, qR);
+    R FL[V], FR[V];
+    flux_axis_split<D>(rf, UL, qL, FL);
+    flux_axis_split<D>(rf, UR, qR, FR);
+    R a = fsqrt(qL.v[0] * qL.v[0] + qL.v[1] * qL.v[1] + qL.v[2] * qL.v[2]) + fsqrt(gam * qL.p * qL.r1);
+    const R aR = fsqrt(qR.v[0] * qR.v[0] + qR.v[1] * qR.v[1] + qR.v[2] * qR.v[2]) + fsqrt(gam * qR.p * qR.r1);
+    if (aR > a) a = aR;
+#pragma unroll
+    for (int s = 0; s < V; ++s) F[s] = nmag * (R(0.5) * (FL[s] + FR[s]) - R(0.5) * a * (UR[s] - UL[s]));
+}
+
+template <int D, class R>
+__device__ __forceinline__ void riemann_hllc_split(R gam, const SplitRef<R>& rf, const R* UL, const R* UR, R nmag,
+                                                   R* F) {
+    SplitPt<R> qL, qR;
+    split_point(gam - R(1), rf, UL, qL);
+    split_point(gam - R(1), rf, UR, qR);
+    const R djac = nmag;
+    const R rhoL = qL.rho, rhoL1 = qL.r1, unL = qL.v[D], PL = qL.p;
+    const R rhoR = qR.rho, rhoR1 = qR.r1, unR = qR.v[D], PR = qR.p;
+    const R aL = fsqrt(gam * PL * rhoL1);
+    const R aR = fsqrt(gam * PR * rhoR1);
+    const R rAvg = R(0.5) * (rhoL + rhoR);
+    const R aAvg = R(0.5) * (aL + aR);
+    const R pStar = fmax(R(0.), R(0.5) * (PL + PR - (qR.dv[D] - qL.dv[D]) * rAvg * aAvg));
+    const R gq = (gam + R(1.)) / (R(2.) * gam);
+    R qsL = R(1.0);
+    const R xL = pStar * frcp(PL);
+    if (xL > R(1.)) qsL = fsqrt(R(1.) + gq * (xL - R(1.)));
+    R qsR = R(1.0);
+    const R xR = pStar * frcp(PR);
+    if (xR > R(1.)) qsR = fsqrt(R(1.) + gq * (xR - R(1.)));
+    const R sL = unL - aL * qsL;
+    const R sR = unR + aR * qsR;
+    const R sStar = ((qR.dp - qL.dp) + rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR)) *
+                    frcp(rhoL * (sL - unL) - rhoR * (sR - unR));
+    R FL[V], FR[V];
+    if (sL >= R(0.)) {
+        flux_axis_split<D>(rf, UL, qL, FL);
+#pragma unroll
+        for (int s = 0; s < V; ++s) F[s] = djac * FL[s];
+    } else if (sR <= R(0.)) {
+        flux_axis_split<D>(rf, UR, qR, FR);
+#pragma unroll
+        for (int s = 0; s < V; ++s) F[s] = djac * FR[s];
+    } else if ((sL < R(0.)) && (sStar >= R(0.))) {
+        // F_L + sL (U*_L - U_L) with U*_L - U_L from the full state
+        flux_axis_split<D>(rf, UL, qL, FL);
+        const R c = (sL - unL) * frcp(sL - sStar);
+        const R f1 = sStar - unL;
+        const R f2 = sStar + PL * frcp(rhoL * (sL - unL));
+        F[0] = djac * (FL[0] + rhoL * sL * (c - R(1.)));
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const R u = (rf.m[i] + UL[1 + i]) * (c - R(1.));
+            F[1 + i] = djac * ((i == D) ? (FL[1 + i] + sL * (u + rhoL * c * f1)) : (FL[1 + i] + sL * u));
+        }
+        F[4] = djac * (FL[4] + sL * ((rf.E + UL[4]) * (c - R(1.)) + rhoL * c * f1 * f2));
+    } else if ((sStar < R(0.)) && (sR > R(0.))) {
+        flux_axis_split<D>(rf, UR, qR, FR);
+        const R c = (sR - unR) * frcp(sR - sStar);
+        const R f1 = sStar - unR;
+        const R f2 = sStar + PR * frcp(rhoR * (sR - unR));
+        F[0] = djac * (FR[0] + rhoR * sR * (c - R(1.)));
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const R u = (rf.m[i] + UR[1 + i]) * (c - R(1.));
+            F[1 + i] = djac * ((i == D) ? (FR[1 + i] + sR * (u + rhoR * c * f1)) : (FR[1 + i] + sR * u));
+        }
+        F[4] = djac * (FR[4] + sR * ((rf.E + UR[4]) * (c - R(1.)) + rhoR * c * f1 * f2));
+    } else {
+#pragma unroll
+        for (int s = 0; s < V; ++s) F[s] = __builtin_nan("");
+    }
+}
+
+template <int D, class R>
+__device__ __forceinline__ void riemann_roe_split(R gam, const SplitRef<R>& rf, const R* UL, const R* UR, R nmag,
+                                                  R* F) {
+    const R gm1 = gam - R(1.);
+    SplitPt<R> qL, qR;
+    split_point(gm1, rf, UL, qL);
+    split_point(gm1, rf, UR, qR);
+    const R rL = qL.rho, rR = qR.rho, r1L = qL.r1, r1R = qR.r1;
+    const R pL = qL.p, pR = qR.p;
+    const R HL = (rf.H + UL[4] + qL.dp) * r1L, HR = (rf.H + UR[4] + qR.dp) * r1R;
+    const R k = fsqrt(rR * r1L);
+    const R wl = frcp(R(1.) + k), wr = k * wl;
+    R ut[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) ut[d] = fma(wl, qL.v[d], wr * qR.v[d]);
+    const R Ht = fma(wl, HL, wr * HR);
+    const R q2 = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
+    const R c2 = gm1 * (Ht - R(0.5) * q2);
+    const R yc = frsq(c2);
+    const R ct = c2 * yc;
+    const R ic2 = yc * yc;
+    const R rt = rL * k;
+    const R unt = ut[D], unL = qL.v[D], unR = qR.v[D];
+    const R drho = UR[0] - UL[0], dp = qR.dp - qL.dp, dun = qR.dv[D] - qL.dv[D];
+    const R a1 = (dp - rt * ct * dun) * (R(0.5) * ic2);
+    const R a5 = (dp + rt * ct * dun) * (R(0.5) * ic2);
+    const R a2 = drho - dp * ic2;
+    const R cL = fsqrt(gam * pL * r1L), cR = fsqrt(gam * pR * r1R);
+    const R l1 = entropy_fix(unt - ct, unL - cL, unR - cR);
+    const R l2 = fabs(unt);
+    const R l5 = entropy_fix(unt + ct, unL + cL, unR + cR);
+    const R w1 = l1 * a1, w5 = l5 * a5, w2 = l2 * a2, ws = l2 * rt;
+    const R wsum = w1 + w5, wdc = (w5 - w1) * ct;
+    R Dv[V];
+    Dv[0] = wsum + w2;
+    R udu = R(0.);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const R du = qR.dv[d] - qL.dv[d];
+        const R dut = (d == D) ? du - dun : du;
+        const R v = fma(Dv[0], ut[d], ws * dut);
+        Dv[1 + d] = (d == D) ? v + wdc : v;
+        udu = fma(ut[d], dut, udu);
+    }
+    Dv[4] = fma(wsum, Ht, fma(wdc, unt, fma(w2 * R(0.5), q2, ws * udu)));
+    R FL[V], FR[V];
+    flux_axis_split<D>(rf, UL, qL, FL);
+    flux_axis_split<D>(rf, UR, qR, FR);
+    const R hn = R(0.5) * nmag;
+#pragma unroll
+    for (int s = 0; s < V; ++s) F[s] = hn * ((FL[s] + FR[s]) - Dv[s]);
+}
+
+template <int FLUX, int D, class R>
+__device__ __forceinline__ void riemann_split(R gam, const SplitRef<R>& rf, const R* UL, const R* UR, R nmag,
+                                              R* F) {
+    if constexpr (FLUX == 0) riemann_llf_split<D>(gam, rf, UL, UR, nmag, F);
+    else if constexpr (FLUX == 1) riemann_hllc_split<D>(gam, rf, UL, UR, nmag, F);
+    else riemann_roe_split<D>(gam, rf, UL, UR, nmag, F);
 }
 
 template <int FLUX, int D, class R>
@@ -1302,7 +1544,10 @@ struct BoxLDS {
     // point data: the three flux vectors [d][c][b][a][s] (15 values), or
     // at p = 5, where those would halve the occupancy, 9 values per point
     static constexpr bool PF15 = P <= 4;
-    static constexpr int QD = PF15 ? 3 * Dm::MMM * V : 9 * Dm::MMM;
+    // values per point of the compact form: 9, or 12 for the FP32 form's
+    // deviation fluxes
+    static constexpr int NQ = sizeof(PR) == 4 ? 12 : 9;
+    static constexpr int QD = PF15 ? 3 * Dm::MMM * V : NQ * Dm::MMM;
     static constexpr int C = 3 * N * M * M * V;
     static constexpr int BV = 2 * N * N * Pd::BV;
     static constexpr int A = cmax(cmax(U + cmax(cmax(F1, FH), L2), X2), cmax(QD, BV));
@@ -1405,8 +1650,8 @@ __device__ __forceinline__ void team_qs(const TaskDec<TS, NR>& dc, int lane, F&&
 
 template <int P, int FLUX, int TS, int NR, class R>
 __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<R>& LT, const KGeom& G,
-                                               const TaskDec<TS, NR>& dc, int lane, const R* sU, R* sR, R* bankA,
-                                               R* bankB) {
+                                               const SplitRef<R>& rf, const TaskDec<TS, NR>& dc, int lane,
+                                               const R* sU, R* sR, R* bankA, R* bankB) {
     using Dm = Dims<P>;
     using L = BoxLDS<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, NNN = Dm::NNN;
@@ -1493,7 +1738,8 @@ __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<
             UR[3] = pr[1 + c2];
             UR[4] = pr[4];
             const R nmag = R(d == 0 ? G.farea[0] : (d == 1 ? G.farea[1] : G.farea[2]));
-            riemann<FLUX, 0>(R(G.gamma), UL, UR, nmag, F);
+            if constexpr (sizeof(R) == 4) riemann_split<FLUX, 0>(R(G.gamma), rf.cycled(d), UL, UR, nmag, F);
+            else riemann<FLUX, 0>(R(G.gamma), UL, UR, nmag, F);
             R* o = fh + ((d * 2 + side) * MM + pt) * V;
             o[0] = F[0];
             o[1 + d] = F[1];
@@ -1516,9 +1762,15 @@ __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<
                 UL[s] = pl[s];
                 UR[s] = pr[s];
             }
-            if (d == 0) riemann<FLUX, 0>(R(G.gamma), UL, UR, R(G.farea[0]), F);
-            else if (d == 1) riemann<FLUX, 1>(R(G.gamma), UL, UR, R(G.farea[1]), F);
-            else riemann<FLUX, 2>(R(G.gamma), UL, UR, R(G.farea[2]), F);
+            if constexpr (sizeof(R) == 4) {
+                if (d == 0) riemann_split<FLUX, 0>(R(G.gamma), rf, UL, UR, R(G.farea[0]), F);
+                else if (d == 1) riemann_split<FLUX, 1>(R(G.gamma), rf, UL, UR, R(G.farea[1]), F);
+                else riemann_split<FLUX, 2>(R(G.gamma), rf, UL, UR, R(G.farea[2]), F);
+            } else {
+                if (d == 0) riemann<FLUX, 0>(R(G.gamma), UL, UR, R(G.farea[0]), F);
+                else if (d == 1) riemann<FLUX, 1>(R(G.gamma), UL, UR, R(G.farea[1]), F);
+                else riemann<FLUX, 2>(R(G.gamma), UL, UR, R(G.farea[2]), F);
+            }
             R* o = fh + ((d * 2 + side) * MM + pt) * V;
 #pragma unroll
             for (int s = 0; s < V; ++s) o[s] = F[s];
@@ -1569,8 +1821,9 @@ __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<
 }
 
 template <int P, int TS, int NR, class R, int PROJ = 0>
-__device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& G, const TaskDec<TS, NR>& dc,
-                                                 int lane, R* sR, R* bankA, R* bankB) {
+__device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& G, const SplitRef<R>& rf,
+                                                 const TaskDec<TS, NR>& dc, int lane, R* sR, R* bankA,
+                                                 R* bankB) {
     static_assert(PROJ == 0 || sizeof(R) == 4, "the bf16 projection belongs to the FP32 form");
     using Dm = Dims<P>;
     constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, MMM = Dm::MMM;
@@ -1628,9 +1881,36 @@ __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& 
     });
     sync();
     // point fluxes: G_d[s] = F_d[s] (dxi_d / dx_d) |J| at each volume point,
-    // F_d[s] = U_s u_d + p delta_{s, d+1} (E + p in place of E)
+    // F_d[s] = U_s u_d + p delta_{s, d+1} (E + p in place of E); the FP32
+    // form holds deviations from the reference state and forms
+    // F_d(Ur + U') - F_d(Ur) = X_s v_d + Y_s v'_d + p' delta_{s, d+1}, with
+    // X = (rho', m', E' + p') and Y = (rho_r, m_r, E_r + p_r)
     constexpr bool PF15 = BoxLDS<P>::PF15;
-    if constexpr (!PF15) {
+    constexpr bool SPLIT = sizeof(R) == 4;
+    constexpr int NQ = BoxLDS<P, 0, R>::NQ;
+    if constexpr (!PF15 && SPLIT) {
+        // (X, g_x v_x, g_y v_y, g_z v_z, g_x v'_x, g_y v'_y, g_z v'_z, p')
+        team_for<MMM, TS>(lane, [&](int pq) {
+            const R* u = Uq + pq * V;
+            R U[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) U[v] = u[v];
+            SplitPt<R> q;
+            split_point(R(G.gamma) - R(1), rf, U, q);
+            R* o = Qd + pq * NQ;
+            o[0] = U[0];
+            o[1] = U[1];
+            o[2] = U[2];
+            o[3] = U[3];
+            o[4] = U[4] + q.dp;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                o[5 + d] = q.v[d] * R(G.gvol[d]);
+                o[8 + d] = q.dv[d] * R(G.gvol[d]);
+            }
+            o[11] = q.dp;
+        });
+    } else if constexpr (!PF15) {
         // (rho, m, E + p, g_x u, g_y v, g_z w, p), g_d = (dxi_d / dx_d) |J|
         team_for<MMM, TS>(lane, [&](int pq) {
             const R* u = Uq + pq * V;
@@ -1648,6 +1928,28 @@ __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& 
 #pragma unroll
             for (int d = 0; d < 3; ++d) o[5 + d] = U[1 + d] * r1 * R(G.gvol[d]);
             o[8] = p;
+        });
+    } else if constexpr (SPLIT) {
+        team_for<MMM, TS>(lane, [&](int pq) {
+            const R* u = Uq + pq * V;
+            R X[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) X[v] = u[v];
+            SplitPt<R> q;
+            split_point(R(G.gamma) - R(1), rf, X, q);
+            X[4] += q.dp;
+            const R Y[V] = {rf.rho, rf.m[0], rf.m[1], rf.m[2], rf.H};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const R g = R(G.gvol[d]);
+                const R vd = q.v[d] * g, dvd = q.dv[d] * g, pd = q.dp * g;
+                R* o = Qd + (d * MMM + pq) * V;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const R f = fma(X[v], vd, Y[v] * dvd);
+                    o[v] = (v == 1 + d) ? f + pd : f;
+                }
+            }
         });
     } else
     team_for<MMM, TS>(lane, [&](int pq) {
@@ -1678,6 +1980,9 @@ __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& 
         // direction s - 1, scaled by that direction's metric
         const R ex = s == 1 ? R(G.gvol[0]) : R(0), ey = s == 2 ? R(G.gvol[1]) : R(0),
                 ez = s == 3 ? R(G.gvol[2]) : R(0);
+        // split 12-value points: Y_s, the reference part of component s
+        const R y0 = rf.rho, y1 = rf.m[0], y2 = rf.m[1], y3 = rf.m[2], y4 = rf.H;
+        const R ys = s == 0 ? y0 : (s == 1 ? y1 : (s == 2 ? y2 : (s == 3 ? y3 : y4)));
 #pragma unroll
         for (int c = 0; c < M; ++c) {
             R gx, gy, gz;
@@ -1686,6 +1991,12 @@ __device__ __forceinline__ void volume_phase_box(const KTables& T, const KGeom& 
                 gx = Qd[o];
                 gy = Qd[MMM * V + o];
                 gz = Qd[2 * MMM * V + o];
+            } else if constexpr (SPLIT) {
+                const R* pt = Qd + (c * MM + q) * NQ;
+                const R Ws = pt[s], pp = pt[11];
+                gx = fma(Ws, pt[5], fma(ys, pt[8], ex * pp));
+                gy = fma(Ws, pt[6], fma(ys, pt[9], ey * pp));
+                gz = fma(Ws, pt[7], fma(ys, pt[10], ez * pp));
             } else {
                 const R* pt = Qd + (c * MM + q) * 9;
                 const R Ws = pt[s], pp = pt[8];
@@ -1990,7 +2301,8 @@ struct StageLoads {
     // neighbour's own slab of that face
     // (FP32 compute: the line rounded to FP32 first, as the own block is)
     template <class R>
-    __device__ __forceinline__ void store_neighbour_slabs(const LaneTabT<R>& LT, R* slab, int lane) const {
+    __device__ __forceinline__ void store_neighbour_slabs(const LaneTabT<R>& LT, const KGeom& G, R* slab,
+                                                          int lane) const {
 #pragma unroll
         for (int r = 0; r < RN; ++r) {
             const int t = r * TS + lane;
@@ -1999,8 +2311,14 @@ struct StageLoads {
                 const int d = ds >> 1, side = ds & 1;
                 const R* phi = side == 0 ? LT.phip : LT.phim;
                 R acc = 0.;
+                if constexpr (sizeof(R) == 4) {
+                    const double ur = uref_of(G, s) * (LT.one[t1] * LT.one[t2]);
 #pragma unroll
-                for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
+                    for (int q = 0; q < N; ++q) acc = fma(phi[q], R(fma(-ur, LT.one[q], nl[r][q])), acc);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
+                }
                 slab[((d * 4 + 2 + side) * N + t2) * BoxPad<P, R>::SL + t1 * V + s] = acc;
             }
         }
@@ -2054,7 +2372,7 @@ struct BoxNbr {
     }
     // the neighbour across the - face contributes its + trace and vice versa
     template <class R>
-    __device__ __forceinline__ void store_slabs(const LaneTabT<R>& LT, R* slab, int lane) const {
+    __device__ __forceinline__ void store_slabs(const LaneTabT<R>& LT, const KGeom& G, R* slab, int lane) const {
         const int wv = __builtin_amdgcn_readfirstlane(lane / WAVE);
         const int l0 = lane % WAVE;
 #pragma unroll
@@ -2066,8 +2384,16 @@ struct BoxNbr {
                     const int d = ds >> 1, side = ds & 1;
                     const R* phi = side == 0 ? LT.phip : LT.phim;
                     R acc = 0.;
+                    if constexpr (sizeof(R) == 4) {
+                        // l = (t2 N + t1) V + s
+                        const int s = l % V, u = l / V;
+                        const double ur = uref_of(G, s) * (LT.one[u % N] * LT.one[u / N]);
 #pragma unroll
-                    for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
+                        for (int q = 0; q < N; ++q) acc = fma(phi[q], R(fma(-ur, LT.one[q], nl[r][q])), acc);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
+                    }
                     // l = (t2 N + t1) V + s: row t2 of the slab, column (t1 V + s)
                     const int t2 = l / (N * V);
                     slab[((d * 4 + 2 + side) * N + t2) * BoxPad<P, R>::SL + (l - t2 * N * V)] = acc;
@@ -2162,6 +2488,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     const R lt_w = tw<R>(T, tl), lt_xv = R(T.xv[tl]);
     const R lt_phim = tphim<R>(T, tp), lt_phip = tphip<R>(T, tp);
     const double lt_md = T.minv[tp][tp];
+    constexpr bool SPLIT = sizeof(R) == 4;
+    const double lt_one = SPLIT ? T.one[tp] : 0.;
     PF pf;
     BoxNbr<P, TS> bn;
     constexpr bool WAVE_NBR = !MAPPED && P >= 2;
@@ -2184,6 +2512,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         LT.phim[lane] = lt_phim;
         LT.phip[lane] = lt_phip;
         LT.mdiag[lane] = lt_md;
+        if constexpr (SPLIT) LT.one[lane] = lt_one;
     }
     if constexpr (MAPPED) {
         team_for<168, TS>(lane, [&](int t) {
@@ -2193,24 +2522,32 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         });
     }
     sync();
+    // FP32 form: the reference state's coefficients (uref_s times the
+    // constant 1 of the basis) come off in FP64, before the rounding
+    const SplitRef<R> rf = split_ref<R>(G);
 #pragma unroll
     for (int r = 0; r < PF::RU; ++r) {
         const int t = r * TS + lane;
         if (t < NB) {
-            bankA[t] = R(pf.own[r]);
+            double x = pf.own[r];
+            if constexpr (SPLIT) {
+                const int b = t / V, s = t - b * V;
+                x -= uref_of(G, s) * (LT.one[b % Dm::N] * LT.one[(b / Dm::N) % Dm::N] * LT.one[b / Dm::NN]);
+            }
+            bankA[t] = R(x);
             sR[t] = R(0);
         }
     }
     if constexpr (MAPPED) pf.store_neighbour_slabs_gen(LT, bankB, lane);
-    else if constexpr (WAVE_NBR) bn.store_slabs(LT, bankB, lane);
-    else pf.store_neighbour_slabs(LT, bankB, lane);
+    else if constexpr (WAVE_NBR) bn.store_slabs(LT, G, bankB, lane);
+    else pf.store_neighbour_slabs(LT, G, bankB, lane);
     sync();
     if constexpr (MAPPED) {
         face_phase<P, FLUX, TS, MAPPED, R>(T, LT, G, lane, bankA, sR, bankA, bankB, XL, e);
         volume_phase<P, TS, MAPPED, R>(T, LT, G, lane, sR, bankA, bankB, XL);
     } else {
-        face_phase_box<P, FLUX, TS>(T, LT, G, dc, lane, bankA, sR, bankA, bankB);
-        volume_phase_box<P, TS, box_rounds<P, TS>(), R, PROJ>(T, G, dc, lane, sR, bankA, bankB);
+        face_phase_box<P, FLUX, TS>(T, LT, G, rf, dc, lane, bankA, sR, bankA, bankB);
+        volume_phase_box<P, TS, box_rounds<P, TS>(), R, PROJ>(T, G, rf, dc, lane, sR, bankA, bankB);
     }
     // FP32 form: the residual widened to FP64 (bank A is free again)
     double* sRd;
@@ -2273,8 +2610,15 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     }
 }
 
+// Waves per SIMD the register allocation leaves room for: the FP32 form of
+// the p = 3 stage is held to 64 VGPRs (8 waves per SIMD, which its LDS
+// allows); elsewhere the compiler's choice.
+template <int P, class R>
+constexpr int stage_min_waves() { return (sizeof(R) == 4 && P == 3) ? 8 : 1; }
+
 template <int P, int FLUX, bool MAPPED, class R = double, int PROJ = 0>
-__global__ __launch_bounds__(stage_team<P>()) void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
+__global__ __launch_bounds__(stage_team<P>()) __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R>())))
+void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
     int ec[3];
     const unsigned e = block_element_xyz(G, ec);

@@ -32,6 +32,7 @@ struct KTables {
     double BF[PMAX + 1][PMAX + 1];  // 2D faces: phi_i at the (p+1)-point segment rule
     double wf[PMAX + 1];            // 2D faces: weights of that rule
     int minv_diag;                  // 1: minv is diagonal (Legendre)
+    double one[PMAX + 1];           // the constant 1 in the 1D basis (Legendre 1, 0, ..; Lagrange all 1)
     // single-precision copies (round to nearest) for the FP32 stage kernel
     float B32[PMAX + 2][PMAX + 1];
     float D32[PMAX + 2][PMAX + 1];
@@ -74,6 +75,11 @@ struct KGeom {
     int fp32;                       // 1: the stage kernel computes in FP32 (box only)
     int proj;                       // FP32 form: x-y projection on bf16 matrix cores,
                                     // 0 none, 1 one bf16 term, 3 hi/lo split (3 terms)
+    // FP32 form: the reference state subtracted from the coefficients (in
+    // FP64) before they are rounded to FP32 -- conserved variables, zero for
+    // none -- with its velocity (uref[1 + d] = uref[0] vref[d] exactly) and
+    // its pressure (gamma - 1)(uref[4] - uref[1..3].vref / 2)
+    double uref[5], vref[3], pref;
     int mapped;                     // 1: the fields below are used
     int exact_normals;              // 1: face normals at the flux points
     const double* X;                // [E][8][3] node coordinates, node 4k+2j+i

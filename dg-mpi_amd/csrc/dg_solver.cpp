@@ -147,6 +147,43 @@ void clear_cmp(dg_solver* s) {
     if (s->cmp) hip_check(dg::launch_zero(s->cmp, s->ncoef * sizeof(float), s->stream), "zero kernel (cmp)");
 }
 
+// Numerics.subtract_mean (FP32 forms): the stage kernels subtract a
+// reference state from the coefficients before rounding them to FP32; it is
+// the domain mean of the current state (conserved by the scheme), rounded
+// so that the kernels' deviation fluxes are consistent: rho_r and v_r are
+// FP32 values, m_r = rho_r v_r exactly in FP64, p_r = (gamma - 1)(E_r -
+// m_r.v_r / 2).  No usable mean (a non-positive density or pressure): no
+// reference.
+void set_reference_state(dg_solver* s) {
+    std::memset(s->geo.uref, 0, sizeof(s->geo.uref));
+    std::memset(s->geo.vref, 0, sizeof(s->geo.vref));
+    s->geo.pref = 0.0;
+    if (!s->cfg.subtract_mean) return;
+    hip_check(dg::launch_eval(s->P, 1, s->time, s->U0, s->work, s->dtab, s->geo, s->ic, s->stream), "eval kernel");
+    std::vector<double> h((size_t)s->geo.E * 6);
+    hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
+              "hipMemcpy");
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    double I[5];
+    for (int v = 0; v < 5; ++v) I[v] = csum(h, s->geo.E, 6, v);
+    const double vol = s->geo.period[0] * s->geo.period[1] * s->geo.period[2];
+    const double rho = (double)(float)(I[0] / vol);
+    if (!(rho > 0.0) || !std::isfinite(rho)) return;
+    double u[5], v[3], mv = 0.0;
+    u[0] = rho;
+    for (int d = 0; d < 3; ++d) {
+        v[d] = (double)(float)(I[1 + d] / I[0]);
+        u[1 + d] = rho * v[d];            // exact: a product of two FP32 values
+        mv += u[1 + d] * v[d];
+    }
+    u[4] = I[4] / vol;
+    const double p = (s->cfg.gamma - 1.0) * (u[4] - 0.5 * mv);
+    if (!(p > 0.0) || !std::isfinite(p) || !std::isfinite(mv)) return;
+    for (int k = 0; k < 5; ++k) s->geo.uref[k] = u[k];
+    for (int d = 0; d < 3; ++d) s->geo.vref[d] = v[d];
+    s->geo.pref = p;
+}
+
 void free_all(dg_solver* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
@@ -413,6 +450,7 @@ int dg_init_state(dg_solver* s) {
         hip_check(dg::launch_init_state(s->P, s->time, s->U0, s->dtab, s->geo, s->ic, s->stream), "init kernel");
         clear_cmp(s);
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        set_reference_state(s);
         if (s->cfg.timestep <= 0.0) {
             // CFL rule (new): dt = cfl * min(h) / max(|u| + c) of the initial state
             const double amax = max_wavespeed(s, s->U0);
@@ -446,6 +484,7 @@ int dg_set_state(dg_solver* s, const double* host, int layout) {
         }
         clear_cmp(s);
         hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        set_reference_state(s);
     });
 }
 

@@ -180,6 +180,17 @@ KTables make_tables(const CaseConfig& c) {
     }
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) t.minv[i][j] = (double)Mq[i][n + j];
+    // coefficients of the constant function 1: M^-1 (integral of phi_j),
+    // in binary128 (1, 0, ... for Legendre; all 1 for Lagrange)
+    for (int i = 0; i < n; ++i) {
+        q128 acc = 0;
+        for (int j = 0; j < n; ++j) {
+            q128 ij = 0;
+            for (int a = 0; a < q; ++a) ij += (q128)t.BQ[a][j] * (q128)w[a];
+            acc += Mq[i][n + j] * ij;
+        }
+        t.one[i] = (double)acc;
+    }
     t.minv_diag = c.basis == Basis::LegendreHex ? 1 : 0;
     if (t.minv_diag)
         for (int i = 0; i < n; ++i)

@@ -349,7 +349,7 @@ class Batch:
 def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(-5., -5., -5.),
               upper=(5., 5., 5.), stepper="RK4", timestep=None, cfl=None, end=1.0, gamma=1.4,
               gas_constant=1.0, ic="ExtrudedIsentropicVortex", ic_data=(1.0, 1.0, 1.0, 1.0, 5.0),
-              warp=0.0, normals="reference", box=True, precision=None, compensated=None):
+              warp=0.0, normals="reference", box=True, precision=None, compensated=None, subtract_mean=None):
     """TOML text of a periodic-box case (keys of the reference input decks;
     warp > 0 gives the non-affine box; two entries in nelem, lower and upper
     give the 2D case, Physics.dim = 2).  box=False: the case of a mesh
@@ -357,11 +357,14 @@ def case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(8, 8, 8), lower=(
     placement and the periodic cell (lower, upper) of the exact solutions.
     precision="FP32": Numerics.precision of the FP32 form (3D box).
     compensated=True/False: Stepper.compensated (default: on for the
-    reduced-precision forms)."""
+    reduced-precision forms).  subtract_mean=True/False:
+    Numerics.subtract_mean (default: on for the reduced-precision forms)."""
     st = f"timestep = {timestep!r}" if timestep is not None else f"cfl = {cfl!r}"
     if compensated is not None:
         st += f"\ncompensated = {'true' if compensated else 'false'}"
     prec = f'\nprecision = "{precision}"' if precision else ""
+    if subtract_mean is not None:
+        prec += f"\nsubtract_mean = {'true' if subtract_mean else 'false'}"
     arr = lambda v: "[" + ", ".join(repr(float(x)) for x in v) + "]"
     # 2: quadrilaterals (basis *Quad, 4 state variables)
     dim = len(nelem) if box else len(lower)

@@ -248,6 +248,14 @@ def test_precision_key():
     assert rc == 1 and "FP16" in msg, msg
     rc, msg = _create(base.replace("warp = 0.0", "warp = 0.1"))
     assert rc == 4 and "precision = FP32 with Mesh.warp" in msg, msg
+    # Numerics.subtract_mean: a boolean, on by default for the reduced
+    # forms, rejected for FP64
+    rc, msg = _create(base.replace('precision = "FP32"', 'precision = "FP32"\nsubtract_mean = false'))
+    assert rc == (0 if hip.device_count() > 0 else 2), msg
+    rc, msg = _create(base.replace('precision = "FP32"', 'precision = "FP32"\nsubtract_mean = 2'))
+    assert rc == 1 and "subtract_mean" in msg, msg
+    rc, msg = _create(base.replace('precision = "FP32"', 'subtract_mean = true'))
+    assert rc == 4 and "subtract_mean" in msg, msg
 
 
 def test_precision_fp32_not_on_2d_or_meshes():

@@ -11,7 +11,15 @@ registers.  Checked against the FP64 oracle:
   exact solution within 1.05x of the FP64 run's, the state within 1e-5;
 * a uniform state stays uniform to FP32 rounding (1e-6); mass and energy
   stay constant to 1e-6 relative (Legendre: the mean mode has no volume
-  term; the face terms cancel to FP32 rounding of the lift).
+  term; the face terms cancel to FP32 rounding of the lift);
+* Numerics.subtract_mean (on by default): the stage subtracts the domain
+  mean from the coefficients before rounding them and forms deviation
+  fluxes F(Ur + U') - F(Ur); the residual of the state set by
+  init_state_from_fcn (whose mean is the reference) is checked against the
+  oracle with the mean subtracted and without, on every flux, both bases and
+  the orders whose code differs (p = 1 and 2: the face-normal-frame
+  solver; p = 3: one solver per direction; p = 5: the 12-value point
+  form).
 """
 import numpy as np
 import pytest
@@ -89,3 +97,28 @@ def test_fp32_free_stream_and_conservation(gpu):
         I = g.integrals()
         assert abs(I[0] - I0[0]) < 1e-6 * abs(I0[0])
         assert abs(I[4] - I0[4]) < 1e-6 * abs(I0[4])
+
+
+@pytest.mark.parametrize("order,basis,flux", [(1, "LegendreHex", "Roe"), (2, "LagrangeHex", "HLLC"),
+                                              (3, "LegendreHex", "Roe"), (3, "LagrangeHex", "LaxFriedrichs"),
+                                              (3, "LegendreHex", "HLLC"), (5, "LegendreHex", "Roe")])
+def test_fp32_mean_subtracted_residual(gpu, order, basis, flux):
+    ne = (4, 3, 5)
+    errs = {}
+    for sub in (False, True):
+        with dg_amd.Solver(dg_amd.case_toml(order=order, basis=basis, flux=flux, nelem=ne, timestep=0.01,
+                                            ic=VORTEX[0], ic_data=VORTEX[1], precision="FP32",
+                                            subtract_mean=sub)) as g:
+            g.init_state_from_fcn()
+            U = g.get_state()
+            o = O.Solver(O.make_params(order=order, basis=basis, flux=flux, nelem=ne, lo=(-5, -5, -5),
+                                       hi=(5, 5, 5), ic=VORTEX[0], ic_data=VORTEX[1]))
+            Ud = hip.Buffer.from_array(U)
+            Rd = hip.Buffer(U.size)
+            g.get_residual(Ud.addr, Rd.addr)
+            hip.synchronize()
+            errs[sub] = relerr(Rd.download(U.shape), o.residual(U))
+    print(f"p={order} {basis} {flux}: FP32 residual rel. error {errs[False]:.2e} plain, "
+          f"{errs[True]:.2e} mean subtracted", flush=True)
+    assert errs[True] < 2e-5 and errs[False] < 2e-5
+    assert errs[True] < errs[False]
