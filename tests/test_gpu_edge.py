@@ -70,3 +70,35 @@ def test_2d_box_with_one_element_along_x():
         g.take_time_step(0.0, 3)
         o.step("RK4", 0.01, 3)
         assert relerr(g.get_state(), o.U) < 1e-11
+
+
+@pytest.mark.parametrize("precision", [None, "FP32"])
+def test_box_past_32_bit_offsets(precision):
+    """192^3 elements at p = 3 (7.1 M elements, 2.3e9 values per register,
+    18 GB each): element offsets e (p+1)^3 5 pass 2^31, so every address of
+    the stage, the initial projection and the reductions must be formed in
+    64 bits.  The z-extruded vortex on [-15, 15]^3 (the headline's element
+    size), two RK4 steps at CFL 0.1: mass, momentum and energy conserved
+    (1e-12 of the largest integral, FP32 form 1e-6) and the L2 error against
+    the exact vortex within three times its t = 0 projection error -- an
+    element read or written at a wrapped offset would put O(1) errors into
+    the free stream beyond it."""
+    ne = 192
+    toml = dg_amd.case_toml(order=3, basis="LegendreHex", flux="Roe", nelem=(ne, ne, ne), lower=(-15.,) * 3,
+                            upper=(15.,) * 3, stepper="RK4", cfl=0.1, end=1.0, ic="ExtrudedIsentropicVortex",
+                            ic_data=(1.0, 1.0, 1.0, 1.0, 5.0), precision=precision)
+    print(f"case 192^3 p3 Roe vortex {precision or 'FP64'}", flush=True)
+    with dg_amd.Solver(toml) as g:
+        assert g.shape[0] * g.shape[1] * g.shape[2] > 2 ** 31
+        g.init_state_from_fcn()
+        e0 = np.array(g.get_error())
+        I0 = np.array(g.integrals())
+        g.take_time_step(g.timestep, 2)
+        e = np.array(g.get_error(g.time))
+        I1 = np.array(g.integrals())
+    print(f"L2 error t=0 {e0}, after 2 steps {e}; integrals {I0[:5]} -> {I1[:5]}", flush=True)
+    assert np.all(np.isfinite(e)) and np.all(np.isfinite(I1))
+    tol = 1e-12 if precision is None else 1e-6
+    assert np.all(np.abs(I1[:5] - I0[:5]) < tol * np.max(np.abs(I0[:5])))
+    v = [0, 1, 2, 4]
+    assert np.all(e[v] < 3.0 * e0[v] + 1e-12)
