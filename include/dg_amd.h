@@ -126,13 +126,19 @@ int dg_get_mesh(const dg_solver* s, long* num_nodes, double* node_coords, long* 
 
 /* L2 projection of the initial condition on (2p+2)^3 Gauss points, time
  * set to 0 (Solver::init_state_from_fcn, src/solver/base.cpp:113-223;
- * SolverTools::L2_projection, src/solver/tools.cpp:66-122). */
+ * SolverTools::L2_projection, src/solver/tools.cpp:66-122).  With
+ * Numerics.subtract_mean (reduced-precision forms) it also sets the
+ * reference state of the FP32 stage: the domain mean of the new state. */
 int dg_init_state(dg_solver* s);
 
 /* Host <-> device copies of the solution coefficients (the h_Uc mirror,
  * src/solver/base.cpp:226-230, and the two layouts of
  * read_in_coefficients, src/solver/base.cpp:233-279).  host holds
- * E*(p+1)^3*5 doubles. */
+ * E*(p+1)^3*5 doubles.  dg_set_state sets the FP32 stage's reference
+ * state as dg_init_state does; a state written through
+ * dg_state_device_ptr keeps the previous one (any reference gives the same
+ * residual; the closer to the state's mean, the smaller its FP32
+ * rounding). */
 int dg_set_state(dg_solver* s, const double* host, int layout);
 int dg_get_state(const dg_solver* s, double* host, int layout);
 
