@@ -2610,11 +2610,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     }
 }
 
-// Waves per SIMD the register allocation leaves room for: the FP32 form of
-// the p = 3 stage is held to 64 VGPRs (8 waves per SIMD, which its LDS
-// allows); elsewhere the compiler's choice.
+// Waves per SIMD the register allocation leaves room for: the FP32 form is
+// held to 64 VGPRs at p <= 3 (8 waves per SIMD, which its LDS allows; at
+// p >= 4 the LDS limits it first); elsewhere the compiler's choice.
 template <int P, class R>
-constexpr int stage_min_waves() { return (sizeof(R) == 4 && P == 3) ? 8 : 1; }
+constexpr int stage_min_waves() { return (sizeof(R) == 4 && P <= 3) ? 8 : 1; }
 
 template <int P, int FLUX, bool MAPPED, class R = double, int PROJ = 0>
 __global__ __launch_bounds__(stage_team<P>()) __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R>())))
@@ -2643,7 +2643,8 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_corr_kernel(StageArgs A
 // arithmetic of its case's own launch.  Positions past the last element
 // (the padding to a multiple of 8) return at once.
 template <int P, int FLUX, class R = double, int TS = stage_team<P>(), int PROJ = 0>
-__global__ __launch_bounds__(TS) void stage_batch_kernel(const BatchCase* __restrict__ cases,
+__global__ __launch_bounds__(TS) __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R>())))
+void stage_batch_kernel(const BatchCase* __restrict__ cases,
                                                                       const unsigned* __restrict__ blk_case,
                                                                       unsigned nreal, int stage,
                                                                       const KTables* __restrict__ Tp,
