@@ -72,7 +72,7 @@ struct LaneTabT {
     R phip[PMAX + 1];        // phi_i(+1)
     double mdiag[PMAX + 1];  // diagonal of the 1D inverse mass matrix
     R xv[PMAX + 2];          // GL nodes of the (p+2)-point rule
-    double one[sizeof(R) == 4 ? PMAX + 1 : 1];   // FP32 form: the constant 1 in the 1D basis
+    double uref[sizeof(R) == 4 ? V : 1];         // FP32 form: the reference state
 };
 using LaneTab = LaneTabT<double>;
 
@@ -2301,7 +2301,7 @@ struct StageLoads {
     // neighbour's own slab of that face
     // (FP32 compute: the line rounded to FP32 first, as the own block is)
     template <class R>
-    __device__ __forceinline__ void store_neighbour_slabs(const LaneTabT<R>& LT, const KGeom& G, R* slab,
+    __device__ __forceinline__ void store_neighbour_slabs(const LaneTabT<R>& LT, int one_kind, R* slab,
                                                           int lane) const {
 #pragma unroll
         for (int r = 0; r < RN; ++r) {
@@ -2312,9 +2312,13 @@ struct StageLoads {
                 const R* phi = side == 0 ? LT.phip : LT.phim;
                 R acc = 0.;
                 if constexpr (sizeof(R) == 4) {
-                    const double ur = uref_of(G, s) * (LT.one[t1] * LT.one[t2]);
+                    // the reference state's coefficients come off before the
+                    // rounding (see stage_body)
+                    const double ur = LT.uref[s];
+                    const double u0 = (one_kind != 0 || (t1 == 0 && t2 == 0)) ? ur : 0.;
+                    const double uq = one_kind != 0 ? ur : 0.;
 #pragma unroll
-                    for (int q = 0; q < N; ++q) acc = fma(phi[q], R(fma(-ur, LT.one[q], nl[r][q])), acc);
+                    for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q] - (q == 0 ? u0 : uq)), acc);
                 } else {
 #pragma unroll
                     for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
@@ -2372,7 +2376,7 @@ struct BoxNbr {
     }
     // the neighbour across the - face contributes its + trace and vice versa
     template <class R>
-    __device__ __forceinline__ void store_slabs(const LaneTabT<R>& LT, const KGeom& G, R* slab, int lane) const {
+    __device__ __forceinline__ void store_slabs(const LaneTabT<R>& LT, int one_kind, R* slab, int lane) const {
         const int wv = __builtin_amdgcn_readfirstlane(lane / WAVE);
         const int l0 = lane % WAVE;
 #pragma unroll
@@ -2385,11 +2389,13 @@ struct BoxNbr {
                     const R* phi = side == 0 ? LT.phip : LT.phim;
                     R acc = 0.;
                     if constexpr (sizeof(R) == 4) {
-                        // l = (t2 N + t1) V + s
-                        const int s = l % V, u = l / V;
-                        const double ur = uref_of(G, s) * (LT.one[u % N] * LT.one[u / N]);
+                        // l = (t2 N + t1) V + s; the reference state's
+                        // coefficients come off before the rounding
+                        const double ur = LT.uref[l % V];
+                        const double u0 = (one_kind != 0 || l < V) ? ur : 0.;
+                        const double uq = one_kind != 0 ? ur : 0.;
 #pragma unroll
-                        for (int q = 0; q < N; ++q) acc = fma(phi[q], R(fma(-ur, LT.one[q], nl[r][q])), acc);
+                        for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q] - (q == 0 ? u0 : uq)), acc);
                     } else {
 #pragma unroll
                         for (int q = 0; q < N; ++q) acc = fma(phi[q], R(nl[r][q]), acc);
@@ -2489,7 +2495,6 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     const R lt_phim = tphim<R>(T, tp), lt_phip = tphip<R>(T, tp);
     const double lt_md = T.minv[tp][tp];
     constexpr bool SPLIT = sizeof(R) == 4;
-    const double lt_one = SPLIT ? T.one[tp] : 0.;
     PF pf;
     BoxNbr<P, TS> bn;
     constexpr bool WAVE_NBR = !MAPPED && P >= 2;
@@ -2512,7 +2517,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         LT.phim[lane] = lt_phim;
         LT.phip[lane] = lt_phip;
         LT.mdiag[lane] = lt_md;
-        if constexpr (SPLIT) LT.one[lane] = lt_one;
+    }
+    if constexpr (SPLIT) {
+        if (lane < V) LT.uref[lane] = uref_of(G, lane);
     }
     if constexpr (MAPPED) {
         team_for<168, TS>(lane, [&](int t) {
@@ -2531,16 +2538,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         if (t < NB) {
             double x = pf.own[r];
             if constexpr (SPLIT) {
-                const int b = t / V, s = t - b * V;
-                x -= uref_of(G, s) * (LT.one[b % Dm::N] * LT.one[(b / Dm::N) % Dm::N] * LT.one[b / Dm::NN]);
+                // coefficient b of the constant: 1 for b = 0 only (Legendre)
+                // or for every b (Lagrange); the host sets no reference for
+                // a basis of another kind
+                const double kap = (T.one_kind != 0 || t < V) ? 1.0 : 0.0;
+                x = fma(-kap, LT.uref[dc.s[r]], x);
             }
             bankA[t] = R(x);
             sR[t] = R(0);
         }
     }
     if constexpr (MAPPED) pf.store_neighbour_slabs_gen(LT, bankB, lane);
-    else if constexpr (WAVE_NBR) bn.store_slabs(LT, G, bankB, lane);
-    else pf.store_neighbour_slabs(LT, G, bankB, lane);
+    else if constexpr (WAVE_NBR) bn.store_slabs(LT, T.one_kind, bankB, lane);
+    else pf.store_neighbour_slabs(LT, T.one_kind, bankB, lane);
     sync();
     if constexpr (MAPPED) {
         face_phase<P, FLUX, TS, MAPPED, R>(T, LT, G, lane, bankA, sR, bankA, bankB, XL, e);

@@ -33,6 +33,7 @@ struct KTables {
     double wf[PMAX + 1];            // 2D faces: weights of that rule
     int minv_diag;                  // 1: minv is diagonal (Legendre)
     double one[PMAX + 1];           // the constant 1 in the 1D basis (Legendre 1, 0, ..; Lagrange all 1)
+    int one_kind;                   // that vector: 0 = (1, 0, ..), 1 = all ones, 2 = otherwise
     // single-precision copies (round to nearest) for the FP32 stage kernel
     float B32[PMAX + 2][PMAX + 1];
     float D32[PMAX + 2][PMAX + 1];

@@ -158,7 +158,10 @@ void set_reference_state(dg_solver* s) {
     std::memset(s->geo.uref, 0, sizeof(s->geo.uref));
     std::memset(s->geo.vref, 0, sizeof(s->geo.vref));
     s->geo.pref = 0.0;
-    if (!s->cfg.subtract_mean) return;
+    // the stage subtracts the reference's coefficients for bases whose
+    // constant function is their first function or the sum of all of them
+    // (Legendre, Lagrange)
+    if (!s->cfg.subtract_mean || s->tab.one_kind == 2) return;
     hip_check(dg::launch_eval(s->P, 1, s->time, s->U0, s->work, s->dtab, s->geo, s->ic, s->stream), "eval kernel");
     std::vector<double> h((size_t)s->geo.E * 6);
     hip_check(hipMemcpyAsync(h.data(), s->work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),

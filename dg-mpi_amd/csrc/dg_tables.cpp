@@ -191,6 +191,14 @@ KTables make_tables(const CaseConfig& c) {
         }
         t.one[i] = (double)acc;
     }
+    {
+        bool delta = std::fabs(t.one[0] - 1.0) < 1e-14, ones = true;
+        for (int i = 0; i < n; ++i) {
+            if (i > 0 && std::fabs(t.one[i]) >= 1e-14) delta = false;
+            if (std::fabs(t.one[i] - 1.0) >= 1e-14) ones = false;
+        }
+        t.one_kind = delta ? 0 : (ones ? 1 : 2);
+    }
     t.minv_diag = c.basis == Basis::LegendreHex ? 1 : 0;
     if (t.minv_diag)
         for (int i = 0; i < n; ++i)
