@@ -2559,18 +2559,22 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
         face_phase_box<P, FLUX, TS>(T, LT, G, rf, dc, lane, bankA, sR, bankA, bankB);
         volume_phase_box<P, TS, box_rounds<P, TS>(), R, PROJ>(T, G, rf, dc, lane, sR, bankA, bankB);
     }
-    // FP32 form: the residual widened to FP64 (bank A is free again)
+    // FP32 form: the residual widened to FP64 in LDS (bank A is free again)
+    // where M^-1 mixes coefficients; a diagonal M^-1 scales each lane's own
+    // entries, which the epilogue widens in registers
     double* sRd;
     if constexpr (sizeof(R) == 8) {
         sRd = sR;
     } else {
         sRd = reinterpret_cast<double*>(bankA);
+        if (!T.minv_diag) {
 #pragma unroll
-        for (int r = 0; r < PF::RU; ++r) {
-            const int t = r * TS + lane;
-            if (t < NB) sRd[t] = (double)sR[t];
+            for (int r = 0; r < PF::RU; ++r) {
+                const int t = r * TS + lane;
+                if (t < NB) sRd[t] = (double)sR[t];
+            }
+            sync();
         }
-        sync();
     }
 
     // The RK registers are read only now: held from the start they would
@@ -2598,7 +2602,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     for (int r = 0; r < PF::RU; ++r) {
         const int t = r * TS + lane;
         if (t < NB) {
-            const double rv = sRd[t];
+            double rv;
+            if constexpr (sizeof(R) == 4) rv = T.minv_diag ? (double)sR[t] : sRd[t];
+            else rv = sRd[t];
             const double sc = (!MAPPED && T.minv_diag) ? scale * lane_diag_scale<P>(LT, t / V) : scale;
             double dU = sc * rv;
             if constexpr (CORR) dU += A.dt * (double)Corr[base + t];
