@@ -1807,15 +1807,26 @@ __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<
         }
     });
     sync();
-    // 6. expand along the normal directions into the residual: +F on the -
-    //    face (the element is the face's right one), -F on the + face
-    team_qs<NNN * V, TS>(dc, lane, [&](int b, int s, int t) {
-        const int i = b % N, j = (b / N) % N, k = b / NN;
-        R acc = sR[t];
-        acc += LT.phim[i] * l2[(0 * N + k) * Pd::L2 + j * V + s] - LT.phip[i] * l2[(1 * N + k) * Pd::L2 + j * V + s];
-        acc += LT.phim[j] * l2[(2 * N + k) * Pd::L2 + i * V + s] - LT.phip[j] * l2[(3 * N + k) * Pd::L2 + i * V + s];
-        acc += LT.phim[k] * l2[(4 * N + j) * Pd::L2 + i * V + s] - LT.phip[k] * l2[(5 * N + j) * Pd::L2 + i * V + s];
-        sR[t] = acc;
+    // 6. expand along the normal directions into the residual, its first
+    //    contribution (the residual starts here): +F on the - face (the
+    //    element is the face's right one), -F on the + face.  One task per
+    //    x line (j, k, s), i along it: the x-face values are common to the
+    //    line, the y- and z-face rows are read at immediate offsets, and
+    //    lanes of one (k, s) or (j, s) read the same words
+    team_qs<NN * V, TS>(dc, lane, [&](int q, int s, int) {
+        const int j = q % N, k = q / N;
+        const R ax = l2[(0 * N + k) * Pd::L2 + j * V + s], bx = l2[(1 * N + k) * Pd::L2 + j * V + s];
+        const R mj = LT.phim[j], pj = LT.phip[j], mk = LT.phim[k], pk = LT.phip[k];
+        const R* ly = l2 + (2 * N + k) * Pd::L2 + s;
+        const R* lz = l2 + (4 * N + j) * Pd::L2 + s;
+        R* out = sR + q * N * V + s;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            R acc = tphim<R>(T, i) * ax - tphip<R>(T, i) * bx;
+            acc += mj * ly[i * V] - pj * ly[N * Pd::L2 + i * V];
+            acc += mk * lz[i * V] - pk * lz[N * Pd::L2 + i * V];
+            out[i * V] = acc;
+        }
     });
     sync();
 }
@@ -2545,7 +2556,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
                 x = fma(-kap, LT.uref[dc.s[r]], x);
             }
             bankA[t] = R(x);
-            sR[t] = R(0);
+            if constexpr (MAPPED) sR[t] = R(0);   // the box path's face phase starts it
         }
     }
     if constexpr (MAPPED) pf.store_neighbour_slabs_gen(LT, bankB, lane);
