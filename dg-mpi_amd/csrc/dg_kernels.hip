@@ -2358,11 +2358,12 @@ struct BoxNbr {
 
     template <int DN>
     __device__ __forceinline__ void line(const double* blk, int l, double* x) const {
-        const int s = l % V, u = l / V, t1 = u % N, t2 = u / N;
+        // l = (t2 N + t1) V + s; the line's first point t1 s1 + t2 s2 + s in
+        // closed form: l + (N - 1) V (l / V) along x, l + (NN - N) V (l / NV)
+        // along y, l along z
         constexpr int sr = DN == 0 ? V : (DN == 1 ? N * V : NN * V);
-        constexpr int s1 = DN == 0 ? N * V : V;
-        constexpr int s2 = DN == 2 ? N * V : NN * V;
-        const double* ln = blk + (t1 * s1 + t2 * s2 + s);
+        const int off = DN == 0 ? l + (N - 1) * V * (l / V) : (DN == 1 ? l + (NN - N) * V * (l / (N * V)) : l);
+        const double* ln = blk + off;
 #pragma unroll
         for (int q = 0; q < N; ++q) x[q] = ln[q * sr];
     }
