@@ -1654,7 +1654,7 @@ __device__ __forceinline__ void face_phase_box(const KTables& T, const LaneTabT<
                                                const R* sU, R* sR, R* bankA, R* bankB) {
     using Dm = Dims<P>;
     using L = BoxLDS<P>;
-    constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM, NNN = Dm::NNN;
+    constexpr int N = Dm::N, M = Dm::M, NN = Dm::NN, MM = Dm::MM;
     R* slab = bankB;
     R* f1 = bankA + L::U;
     R* f2 = bankB;
@@ -2638,14 +2638,21 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     }
 }
 
-// Waves per SIMD the register allocation leaves room for: the FP32 form is
-// held to 64 VGPRs at p <= 3 (8 waves per SIMD, which its LDS allows; at
-// p >= 4 the LDS limits it first); elsewhere the compiler's choice.
-template <int P, class R>
-constexpr int stage_min_waves() { return (sizeof(R) == 4 && P <= 3) ? 8 : 1; }
+// Waves per SIMD the register allocation leaves room for: the FP32 forms
+// are held to 64 VGPRs (8 waves per SIMD) at p <= 3 where their LDS allows
+// 8 waves of TS-thread workgroups (not at p >= 4, nor for BF16x3 at p = 3,
+// whose matrix-core accumulators then need registers beyond 64); elsewhere
+// the compiler's choice.
+template <int P, class R, int TS, int PROJ>
+constexpr int stage_min_waves() {
+    constexpr int lds = BoxLDS<P, proj_bank<P, PROJ, R>(), R>::TOTAL * (int)sizeof(R) + 512;
+    constexpr int waves = (160 * 1024 / lds) * (TS / WAVE) / 4;
+    return (sizeof(R) == 4 && P <= 3 && !(PROJ == 3 && P == 3) && waves >= 8) ? 8 : 1;
+}
 
 template <int P, int FLUX, bool MAPPED, class R = double, int PROJ = 0>
-__global__ __launch_bounds__(stage_team<P>()) __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R>())))
+__global__ __launch_bounds__(stage_team<P>())
+__attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R, stage_team<P>(), PROJ>())))
 void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
     int ec[3];
@@ -2671,7 +2678,7 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_corr_kernel(StageArgs A
 // arithmetic of its case's own launch.  Positions past the last element
 // (the padding to a multiple of 8) return at once.
 template <int P, int FLUX, class R = double, int TS = stage_team<P>(), int PROJ = 0>
-__global__ __launch_bounds__(TS) __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R>())))
+__global__ __launch_bounds__(TS) __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R, TS, PROJ>())))
 void stage_batch_kernel(const BatchCase* __restrict__ cases,
                                                                       const unsigned* __restrict__ blk_case,
                                                                       unsigned nreal, int stage,
