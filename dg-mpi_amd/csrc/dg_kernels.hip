@@ -390,6 +390,7 @@ __device__ __forceinline__ void riemann_roe(R gam, const R* UL, const R* UR, R n
 // flux cancel under the (p+2)-point rule), so the residual is unchanged,
 // while FP32 rounding scales with the deviation U' instead of the state.
 // Ur = 0 gives the plain fluxes.
+
 // uref[s] of the reference state by selects on values (a dynamic index,
 // or a select of the members' addresses, would copy the kernel-argument
 // struct to scratch memory)
