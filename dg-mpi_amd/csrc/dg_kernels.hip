@@ -2471,7 +2471,8 @@ __device__ __forceinline__ void dense_minv(const double* __restrict__ imm, int l
 // which read and write the FP64 registers.  Periodic box only.
 // CORR: the RK update takes dU = dt (M^-1 R + C), C a per-DOF correction in
 // FP32 laid out like the state (the learned closure's corrected stage)
-template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>(), bool CORR = false, int PROJ = 0>
+template <int P, int FLUX, bool MAPPED, class R, int TS = stage_team<P>(), bool CORR = false, int PROJ = 0,
+          int MODE = -1>
 __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T, const KGeom& G, unsigned e,
                                            const int* ec, const float* __restrict__ Corr = nullptr) {
     static_assert(sizeof(R) == 8 || !MAPPED, "the FP32 form is for the periodic box");
@@ -2493,7 +2494,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     R* bankA = lds + L::R;
     R* bankB = bankA + LA;
     const int lane = threadIdx.x;
-    const bool rk = A.mode != MODE_RESIDUAL;
+    const bool rk = (MODE >= 0 ? MODE : A.mode) != MODE_RESIDUAL;
     const TaskDec<TS, box_rounds<P, TS>()> dc(lane);
 
     // Every global load of the element at once: the lane-table entries
@@ -2600,7 +2601,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const KTables& T,
     // mode, the backend has merged the stores into one block whose base
     // register was left unset on the forward-Euler path.
     const size_t base = (size_t)e * NB;
-    const int mode = A.mode;
+    const int mode = MODE >= 0 ? MODE : A.mode;
     if constexpr (MAPPED) {
         if (rk) dense_minv<P, TS>(G.imm + (size_t)e * Dm::NNN * Dm::NNN, lane, sR, bankA);
     } else {
@@ -2651,14 +2652,14 @@ constexpr int stage_min_waves() {
     return (sizeof(R) == 4 && P <= 3 && !(PROJ == 3 && P == 3) && waves >= 8) ? 8 : 1;
 }
 
-template <int P, int FLUX, bool MAPPED, class R = double, int PROJ = 0>
+template <int P, int FLUX, bool MAPPED, class R = double, int PROJ = 0, int MODE = -1>
 __global__ __launch_bounds__(stage_team<P>())
 __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R, stage_team<P>(), PROJ>())))
 void stage_kernel(StageArgs A, const KTables* __restrict__ Tp,
                                                                 KGeom G) {
     int ec[3];
     const unsigned e = block_element_xyz(G, ec);
-    stage_body<P, FLUX, MAPPED, R, stage_team<P>(), false, PROJ>(A, *Tp, G, e, ec);
+    stage_body<P, FLUX, MAPPED, R, stage_team<P>(), false, PROJ, MODE>(A, *Tp, G, e, ec);
 }
 
 // The closure's corrected RK stage on an affine FP64 box (RK modes only)
@@ -3065,6 +3066,19 @@ __global__ __launch_bounds__(256) void mass_kernel(double* imm, double* jmin, co
 // ---------------------------------------------------------------------------
 // Dispatch
 // ---------------------------------------------------------------------------
+// FP64 box stage with the RK mode compiled in (its epilogue then has no
+// mode selects); the residual mode keeps the run-time form
+template <int P, int FLUX>
+void stage_fp64_box(const StageArgs& a, const KTables* t, const KGeom& g, dim3 grid, dim3 block, hipStream_t st) {
+    switch (a.mode) {
+    case MODE_RK_FIRST: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_RK_FIRST>), grid, block, 0, st, a, t, g); break;
+    case MODE_RK_MID: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_RK_MID>), grid, block, 0, st, a, t, g); break;
+    case MODE_RK_LAST: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_RK_LAST>), grid, block, 0, st, a, t, g); break;
+    case MODE_FE: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_FE>), grid, block, 0, st, a, t, g); break;
+    default: hipLaunchKernelGGL((stage_kernel<P, FLUX, false>), grid, block, 0, st, a, t, g); break;
+    }
+}
+
 template <int P>
 hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& g, hipStream_t st) {
     dim3 grid(g.E), block(stage_team<P>());
@@ -3099,9 +3113,9 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
         }
     } else {
         switch (flux) {
-        case 0: hipLaunchKernelGGL((stage_kernel<P, 0, false>), grid, block, 0, st, a, t, g); break;
-        case 1: hipLaunchKernelGGL((stage_kernel<P, 1, false>), grid, block, 0, st, a, t, g); break;
-        default: hipLaunchKernelGGL((stage_kernel<P, 2, false>), grid, block, 0, st, a, t, g); break;
+        case 0: stage_fp64_box<P, 0>(a, t, g, grid, block, st); break;
+        case 1: stage_fp64_box<P, 1>(a, t, g, grid, block, st); break;
+        default: stage_fp64_box<P, 2>(a, t, g, grid, block, st); break;
         }
     }
     return hipGetLastError();
