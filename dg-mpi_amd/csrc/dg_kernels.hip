@@ -3066,16 +3066,17 @@ __global__ __launch_bounds__(256) void mass_kernel(double* imm, double* jmin, co
 // ---------------------------------------------------------------------------
 // Dispatch
 // ---------------------------------------------------------------------------
-// FP64 box stage with the RK mode compiled in (its epilogue then has no
-// mode selects); the residual mode keeps the run-time form
-template <int P, int FLUX>
-void stage_fp64_box(const StageArgs& a, const KTables* t, const KGeom& g, dim3 grid, dim3 block, hipStream_t st) {
+// Box stage (FP64 or the FP32 form) with the RK mode compiled in (its
+// epilogue then has no mode selects); the residual mode keeps the run-time
+// form
+template <int P, int FLUX, class R>
+void stage_box_mode(const StageArgs& a, const KTables* t, const KGeom& g, dim3 grid, dim3 block, hipStream_t st) {
     switch (a.mode) {
-    case MODE_RK_FIRST: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_RK_FIRST>), grid, block, 0, st, a, t, g); break;
-    case MODE_RK_MID: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_RK_MID>), grid, block, 0, st, a, t, g); break;
-    case MODE_RK_LAST: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_RK_LAST>), grid, block, 0, st, a, t, g); break;
-    case MODE_FE: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, double, 0, MODE_FE>), grid, block, 0, st, a, t, g); break;
-    default: hipLaunchKernelGGL((stage_kernel<P, FLUX, false>), grid, block, 0, st, a, t, g); break;
+    case MODE_RK_FIRST: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, R, 0, MODE_RK_FIRST>), grid, block, 0, st, a, t, g); break;
+    case MODE_RK_MID: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, R, 0, MODE_RK_MID>), grid, block, 0, st, a, t, g); break;
+    case MODE_RK_LAST: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, R, 0, MODE_RK_LAST>), grid, block, 0, st, a, t, g); break;
+    case MODE_FE: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, R, 0, MODE_FE>), grid, block, 0, st, a, t, g); break;
+    default: hipLaunchKernelGGL((stage_kernel<P, FLUX, false, R>), grid, block, 0, st, a, t, g); break;
     }
 }
 
@@ -3107,15 +3108,15 @@ hipError_t stage_p(int flux, const StageArgs& a, const KTables* t, const KGeom& 
         }
     } else if (g.fp32) {
         switch (flux) {
-        case 0: hipLaunchKernelGGL((stage_kernel<P, 0, false, float>), grid, block, 0, st, a, t, g); break;
-        case 1: hipLaunchKernelGGL((stage_kernel<P, 1, false, float>), grid, block, 0, st, a, t, g); break;
-        default: hipLaunchKernelGGL((stage_kernel<P, 2, false, float>), grid, block, 0, st, a, t, g); break;
+        case 0: stage_box_mode<P, 0, float>(a, t, g, grid, block, st); break;
+        case 1: stage_box_mode<P, 1, float>(a, t, g, grid, block, st); break;
+        default: stage_box_mode<P, 2, float>(a, t, g, grid, block, st); break;
         }
     } else {
         switch (flux) {
-        case 0: stage_fp64_box<P, 0>(a, t, g, grid, block, st); break;
-        case 1: stage_fp64_box<P, 1>(a, t, g, grid, block, st); break;
-        default: stage_fp64_box<P, 2>(a, t, g, grid, block, st); break;
+        case 0: stage_box_mode<P, 0, double>(a, t, g, grid, block, st); break;
+        case 1: stage_box_mode<P, 1, double>(a, t, g, grid, block, st); break;
+        default: stage_box_mode<P, 2, double>(a, t, g, grid, block, st); break;
         }
     }
     return hipGetLastError();
