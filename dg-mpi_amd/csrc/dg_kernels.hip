@@ -2679,7 +2679,7 @@ __global__ __launch_bounds__(stage_team<P>()) void stage_corr_kernel(StageArgs A
 // tables and RK registers (BatchCase), so every element sees the stage
 // arithmetic of its case's own launch.  Positions past the last element
 // (the padding to a multiple of 8) return at once.
-template <int P, int FLUX, class R = double, int TS = stage_team<P>(), int PROJ = 0>
+template <int P, int FLUX, class R = double, int TS = stage_team<P>(), int PROJ = 0, int MODE = -1>
 __global__ __launch_bounds__(TS) __attribute__((amdgpu_waves_per_eu(stage_min_waves<P, R, TS, PROJ>())))
 void stage_batch_kernel(const BatchCase* __restrict__ cases,
                                                                       const unsigned* __restrict__ blk_case,
@@ -2709,7 +2709,7 @@ void stage_batch_kernel(const BatchCase* __restrict__ cases,
     const KGeom& G = C->g;
     int ec[3];
     const unsigned e = brick_element_xyz(G, q - C->first_block, ec);
-    stage_body<P, FLUX, false, R, TS, false, PROJ>(A, *Tp, G, e, ec);
+    stage_body<P, FLUX, false, R, TS, false, PROJ, MODE>(A, *Tp, G, e, ec);
 }
 
 // dU = dt M^-1 R
@@ -3135,24 +3135,60 @@ hipError_t stage_batch_proj(int flux, const BatchCase* cases, const unsigned* bl
     return hipGetLastError();
 }
 
+// batched stage with the RK mode of the stage compiled in (stages 1 and 2
+// share the middle mode), as the single-case box stage has it
+template <int P, int FLUX, class R, int TS>
+void stage_batch_mode(const BatchCase* cases, const unsigned* blk_case, unsigned nblocks, int stage,
+                      const KTables* t, dim3 grid, dim3 block, hipStream_t st) {
+    const char* base = reinterpret_cast<const char*>(cases);
+    if (stage == 0)
+        hipLaunchKernelGGL((stage_batch_kernel<P, FLUX, R, TS, 0, MODE_RK_FIRST>), grid, block, 0, st, cases, blk_case,
+                           nblocks, stage, t, base);
+    else if (stage == 3)
+        hipLaunchKernelGGL((stage_batch_kernel<P, FLUX, R, TS, 0, MODE_RK_LAST>), grid, block, 0, st, cases, blk_case,
+                           nblocks, stage, t, base);
+    else
+        hipLaunchKernelGGL((stage_batch_kernel<P, FLUX, R, TS, 0, MODE_RK_MID>), grid, block, 0, st, cases, blk_case,
+                           nblocks, stage, t, base);
+}
+
 template <int P, int TS>
 hipError_t stage_batch_pt(int flux, int fp32, const BatchCase* cases, const unsigned* blk_case, unsigned nblocks,
                           int stage, const KTables* t, hipStream_t st) {
     const dim3 grid((nblocks + 7u) & ~7u), block(TS);
-    if (fp32) {
-        switch (flux) {
-        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+    if constexpr (TS == stage_team<P>()) {
+        // the default team: the RK mode compiled in
+        if (fp32) {
+            switch (flux) {
+            case 0: stage_batch_mode<P, 0, float, TS>(cases, blk_case, nblocks, stage, t, grid, block, st); break;
+            case 1: stage_batch_mode<P, 1, float, TS>(cases, blk_case, nblocks, stage, t, grid, block, st); break;
+            default: stage_batch_mode<P, 2, float, TS>(cases, blk_case, nblocks, stage, t, grid, block, st); break;
+            }
+        } else {
+            switch (flux) {
+            case 0: stage_batch_mode<P, 0, double, TS>(cases, blk_case, nblocks, stage, t, grid, block, st); break;
+            case 1: stage_batch_mode<P, 1, double, TS>(cases, blk_case, nblocks, stage, t, grid, block, st); break;
+            default: stage_batch_mode<P, 2, double, TS>(cases, blk_case, nblocks, stage, t, grid, block, st); break;
+            }
         }
+        return hipGetLastError();
     } else {
-        switch (flux) {
-        case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
-        default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+        // the second team (tuning candidate): the run-time mode
+        if (fp32) {
+            switch (flux) {
+            case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+            case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+            default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, float, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+            }
+        } else {
+            switch (flux) {
+            case 0: hipLaunchKernelGGL((stage_batch_kernel<P, 0, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+            case 1: hipLaunchKernelGGL((stage_batch_kernel<P, 1, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+            default: hipLaunchKernelGGL((stage_batch_kernel<P, 2, double, TS>), grid, block, 0, st, cases, blk_case, nblocks, stage, t, reinterpret_cast<const char*>(cases)); break;
+            }
         }
+        return hipGetLastError();
     }
-    return hipGetLastError();
 }
 
 // the second team size the batched launch can run each order with (the
